@@ -57,7 +57,12 @@ typedef cl_bitfield cl_mem_flags;
 #  define CL_MEM_READ_ONLY            (1 << 2)
 #endif
 
-#define ocip_API
+/* Only the ocip* entry points are exported from the shared library. */
+#if defined(OCIP_BUILD)
+#  define ocip_API __attribute__((visibility("default")))
+#else
+#  define ocip_API
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -167,6 +172,13 @@ ocipError ocip_API ocipCrossCorr_NormBackward(ocipImage Source, ocipImage Templa
 ocipError ocip_API ocipSqrDistanceBackward(ocipImage Source, ocipImage Template,
                                            ocipImage GradDest, ocipImage GradSource,
                                            ocipImage GradTemplate);
+
+/* Kernel timing (profiling aid).  While enabled, every kernel the library launches is
+ * bracketed by HIP events on the stream it runs on.  ocipGetKernelTime waits for the queued
+ * work and returns the total time (ms) and launch count of the kernels whose name starts
+ * with KernelName since timing was last enabled. */
+ocipError ocip_API ocipEnableKernelTiming(ocipBool Enable);
+ocipError ocip_API ocipGetKernelTime(const char * KernelName, double * TotalMs, uint * Count);
 
 #ifdef __cplusplus
 }

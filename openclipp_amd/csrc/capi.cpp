@@ -1,0 +1,491 @@
+// capi.cpp -- the extern "C" boundary of libOpenCLIPP.so.
+//
+// Every reference entry point keeps its name, arguments, asynchrony and error codes
+// (include/OpenCLIPP.h of the reference; C layer OpenCLIPP/OpenCLIPP.cpp:45-728).  Errors
+// raised below the ABI are ocip::Error values; the ABI returns their code, as the
+// reference's H() macro returns cl::Error::err() (OpenCLIPP/OpenCLIPP.cpp:82).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "ocip_internal.h"
+#include "validation.h"
+
+using namespace ocip;
+
+namespace {
+
+// Current context: process-global and unguarded, as in the reference
+// (OpenCLIPP/OpenCLIPP.cpp:73-75).  The ABI is not thread-safe.
+Context *g_current = nullptr;
+// Contexts not yet released: an image released after its context is freed only returns its
+// memory (the reference leaves that case undefined).
+std::vector<Context *> g_live;
+
+bool live(Context *c)
+{
+   for (Context *p : g_live)
+      if (p == c) return true;
+   return false;
+}
+
+template <class F> cl_int guarded(F &&f)
+{
+   try {
+      f();
+   } catch (const Error &e) {
+      return e.code;
+   } catch (const std::bad_alloc &) {
+      return CL_OUT_OF_HOST_MEMORY;
+   }
+   return CL_SUCCESS;
+}
+
+Context &current()
+{
+   if (!g_current) fail(CL_INVALID_CONTEXT, "no current context");
+   return *g_current;
+}
+
+void set_device(const Context &c) { hip_check(hipSetDevice(c.device), "hipSetDevice"); }
+
+Image &img(ocipImage h)
+{
+   Image *p = reinterpret_cast<Image *>(h);
+   if (!p || p->magic != kImageMagic) fail(CL_INVALID_MEM_OBJECT, "not an image");
+   return *p;
+}
+
+Program &prog(ocipProgram h, int kind)
+{
+   Program *p = reinterpret_cast<Program *>(h);
+   if (!p || p->magic != kProgramMagic || p->kind != kind) fail(CL_INVALID_PROGRAM, "bad program");
+   return *p;
+}
+
+View view(const Image &i)
+{
+   return View{i.dev, (int)i.desc.Width, (int)i.desc.Height, (int)i.desc.Step,
+               (int)i.desc.Channels, (int)i.desc.Type};
+}
+
+// Lazy host->device transfer of an input (reference kernel_helpers.h:135, Buffer.cpp:190-194)
+void send_if_needed(Context &c, Image &i)
+{
+   if (i.in_device) return;
+   if (i.host)
+      hip_check(hipMemcpyAsync(i.dev, i.host, i.bytes, hipMemcpyHostToDevice, c.stream),
+                "send");
+   i.in_device = true;
+}
+
+// Images must belong to the current context: kernels run on its stream.
+void same_context(Context &c, const Image &i)
+{
+   if (i.ctx != &c) fail(CL_INVALID_CONTEXT, "image from another context");
+}
+
+void check_same_size(const Image &a, const Image &b)
+{
+   if (a.desc.Width != b.desc.Width || a.desc.Height != b.desc.Height)
+      fail(CL_INVALID_IMAGE_SIZE, "different image sizes");
+}
+
+void throw_if(cl_int code, const char *what)
+{
+   if (code != CL_SUCCESS) fail(code, what);
+}
+
+// ---- proximity driver ---------------------------------------------------------------------
+size_t integ_bytes(const Image &S)
+{
+   return (size_t)(S.desc.Width + 1) * (S.desc.Height + 1) * sizeof(double);
+}
+
+// FP64 exclusive sum-of-squares integral of channel c of S into c.integ.
+const double *sqr_integral_for(Context &c, const Image &S, int channel)
+{
+   View v = view(S);
+   c.integ.ensure(integ_bytes(S));
+   c.integ_aux.ensure(integral_aux_bytes(v.width, v.height));
+   launch_integral(c.stream, v, channel, true, c.integ.ptr, (v.width + 1) * (int)sizeof(double),
+                   -1, c.integ_aux.ptr);
+   return c.integ.as<double>();
+}
+
+int tpitch_for(int tw) { return (tw + 15) / 16 * 16; }
+
+void run_proximity(int op, ocipImage hs, ocipImage ht, ocipImage hd, int precision)
+{
+   Context &c = current();
+   Image &S = img(hs), &T = img(ht), &D = img(hd);
+   throw_if(check::proximity(S.desc, T.desc, D.desc), "proximity arguments");
+   throw_if(check::precision(S.desc, precision), "precision");
+   same_context(c, S); same_context(c, T); same_context(c, D);
+   set_device(c);
+   send_if_needed(c, S);
+   send_if_needed(c, T);
+
+   const int tw = T.desc.Width, th = T.desc.Height, nch = S.desc.Channels;
+   const int type = S.desc.Type;
+   const bool norm = op == kSSDNorm || op == kCCNorm;
+   // compute type of the staged template: double for F64, int32 for 8/16-bit integers
+   // (exact integer sums), float otherwise.
+   const int kind = type == SImage::F64 ? 1 : (elem_size(type) <= 2 ? 2 : 0);
+   const int esz = kind == 1 ? 8 : 4;
+   const int tpitch = tpitch_for(tw);
+   c.tmpl.data.ensure((size_t)tpitch * (th > 0 ? th : 1) * esz);
+   c.tmpl.sums.ensure(2 * sizeof(double));
+
+   for (int ch = 0; ch < nch; ++ch) {
+      const double *integ = norm ? sqr_integral_for(c, S, ch) : nullptr;
+      launch_template_prep(c.stream, view(T), ch, tpitch, kind, c.tmpl.data.ptr,
+                           c.tmpl.sums.as<double>());
+      if (precision != ocipPrecisionF32) {
+         size_t wb = crosscorr_mfma_work_bytes(precision, S.desc.Width, S.desc.Height, tw, th);
+         c.stats.ensure(wb > 0 ? wb : 1);
+         launch_crosscorr_mfma(c.stream, precision, norm, view(S), tw, th,
+                               c.tmpl.data.as<float>(), tpitch, c.tmpl.sums.as<double>(),
+                               integ, (float *)D.dev, D.desc.Step, c.stats.ptr);
+      } else {
+         launch_proximity(c.stream, op, view(S), view(T), D.dev, D.desc.Step, c.tmpl.data.ptr,
+                          tpitch, c.tmpl.sums.as<double>(), integ, ch);
+      }
+   }
+   D.in_device = true;
+}
+
+}  // namespace
+
+void DevBuf::ensure(size_t n)
+{
+   if (n <= bytes && ptr) return;
+   release();
+   void *p = nullptr;
+   hip_check(hipMalloc(&p, n ? n : 1), "workspace allocation");
+   ptr = p;
+   bytes = n;
+}
+
+void DevBuf::release()
+{
+   if (ptr) {
+      // the buffer may still be read by queued kernels
+      (void)hipDeviceSynchronize();
+      (void)hipFree(ptr);
+   }
+   ptr = nullptr;
+   bytes = 0;
+}
+
+// ======================================================================= extern "C"
+extern "C" {
+
+ocipError ocip_API ocipInitialize(ocipContext *ContextPtr, const char *PreferredPlatform,
+                                  cl_device_type deviceType)
+{
+   (void)PreferredPlatform;
+   return guarded([&] {
+      if (!ContextPtr) fail(CL_INVALID_VALUE, "null context pointer");
+      const cl_device_type gpu_ok = CL_DEVICE_TYPE_GPU | CL_DEVICE_TYPE_DEFAULT;
+      if (deviceType != (cl_device_type)CL_DEVICE_TYPE_ALL && !(deviceType & gpu_ok))
+         fail(CL_DEVICE_NOT_FOUND, "only GPU devices");
+      int n = 0;
+      if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) fail(CL_DEVICE_NOT_FOUND, "no GPU");
+      Context *c = new Context();
+      c->device = 0;  // devices[0], as the reference (C++/OpenCL.cpp:57-62)
+      try {
+         hip_check(hipSetDevice(c->device), "hipSetDevice");
+         hipDeviceProp_t p;
+         hip_check(hipGetDeviceProperties(&p, c->device), "properties");
+         c->name = p.name;
+         if (p.gcnArchName[0]) c->name += std::string(" (") + p.gcnArchName + ")";
+         c->num_cus = p.multiProcessorCount;
+         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream");
+      } catch (...) {
+         delete c;
+         throw;
+      }
+      g_current = c;
+      g_live.push_back(c);
+      *ContextPtr = reinterpret_cast<ocipContext>(c);
+   });
+}
+
+ocipError ocip_API ocipUninitialize(ocipContext Context)
+{
+   ocip::Context *c = reinterpret_cast<ocip::Context *>(Context);
+   if (!c || c->magic != kContextMagic || !live(c)) return CL_INVALID_CONTEXT;
+   if (c == g_current) g_current = nullptr;
+   for (size_t k = 0; k < g_live.size(); ++k)
+      if (g_live[k] == c) { g_live.erase(g_live.begin() + k); break; }
+   return guarded([&] {
+      (void)hipSetDevice(c->device);
+      (void)hipStreamSynchronize(c->stream);
+      c->tmpl.data.release();
+      c->tmpl.sums.release();
+      c->integ.release();
+      c->integ_aux.release();
+      c->stats.release();
+      (void)hipStreamDestroy(c->stream);
+      c->magic = 0;
+      delete c;
+   });
+}
+
+ocipError ocip_API ocipChangeContext(ocipContext Context)
+{
+   ocip::Context *c = reinterpret_cast<ocip::Context *>(Context);
+   if (!c || !live(c)) return CL_INVALID_CONTEXT;
+   g_current = c;
+   return CL_SUCCESS;
+}
+
+void ocip_API ocipSetCLFilesPath(const char *Path) { (void)Path; }
+
+ocip_API const char *ocipGetErrorName(ocipError Error)
+{
+   switch (Error) {
+   case CL_SUCCESS: return "SUCCESS";
+   case CL_DEVICE_NOT_FOUND: return "CL_DEVICE_NOT_FOUND";
+   case CL_DEVICE_NOT_AVAILABLE: return "CL_DEVICE_NOT_AVAILABLE";
+   case CL_MEM_OBJECT_ALLOCATION_FAILURE: return "CL_MEM_OBJECT_ALLOCATION_FAILURE";
+   case CL_OUT_OF_RESOURCES: return "CL_OUT_OF_RESOURCES";
+   case CL_OUT_OF_HOST_MEMORY: return "CL_OUT_OF_HOST_MEMORY";
+   case CL_IMAGE_FORMAT_NOT_SUPPORTED: return "CL_IMAGE_FORMAT_NOT_SUPPORTED";
+   case CL_INVALID_VALUE: return "CL_INVALID_VALUE";
+   case CL_INVALID_DEVICE_TYPE: return "CL_INVALID_DEVICE_TYPE";
+   case CL_INVALID_CONTEXT: return "CL_INVALID_CONTEXT";
+   case CL_INVALID_MEM_OBJECT: return "CL_INVALID_MEM_OBJECT";
+   case CL_INVALID_IMAGE_SIZE: return "CL_INVALID_IMAGE_SIZE";
+   case CL_INVALID_PROGRAM: return "CL_INVALID_PROGRAM";
+   case CL_INVALID_OPERATION: return "CL_INVALID_OPERATION";
+   case CL_INVALID_BUFFER_SIZE: return "CL_INVALID_BUFFER_SIZE";
+   default: return "Unknown error";
+   }
+}
+
+ocipError ocip_API ocipGetDeviceName(char *Name, uint BufferLength)
+{
+   if (!g_current) return CL_INVALID_CONTEXT;
+   const std::string &n = g_current->name;
+   if (!Name || BufferLength == 0 || n.size() > BufferLength - 1) return CL_OUT_OF_RESOURCES;
+   memcpy(Name, n.c_str(), n.size() + 1);
+   return CL_SUCCESS;
+}
+
+ocipError ocip_API ocipFinish(void)
+{
+   if (!g_current) return CL_INVALID_CONTEXT;
+   return guarded([&] {
+      set_device(*g_current);
+      hip_check(hipStreamSynchronize(g_current->stream), "finish");
+   });
+}
+
+ocipError ocip_API ocipCreateImage(ocipImage *ImagePtr, SImage Image, void *ImageData,
+                                   cl_mem_flags flags)
+{
+   if (!g_current) return CL_INVALID_CONTEXT;
+   return guarded([&] {
+      if (!ImagePtr) fail(CL_INVALID_VALUE, "null image pointer");
+      throw_if(check::create_image(Image), "image description");
+      const cl_mem_flags known = CL_MEM_READ_WRITE | CL_MEM_WRITE_ONLY | CL_MEM_READ_ONLY;
+      if (flags & ~known) fail(CL_INVALID_VALUE, "flags");
+      size_t bytes = (size_t)Image.Height * Image.Step;
+      set_device(*g_current);
+      ocip::Image *i = new ocip::Image();
+      i->ctx = g_current;
+      i->desc = Image;
+      i->host = ImageData;
+      i->bytes = bytes;
+      if (hipMalloc(&i->dev, bytes) != hipSuccess) {
+         delete i;
+         fail(CL_MEM_OBJECT_ALLOCATION_FAILURE, "image allocation");
+      }
+      *ImagePtr = reinterpret_cast<ocipImage>(i);
+   });
+}
+
+ocipError ocip_API ocipSendImage(ocipImage Image)
+{
+   return guarded([&] {
+      ocip::Image &i = img(Image);
+      if (!i.host) return;  // device-only image (C++/Buffer.cpp:158-159)
+      set_device(*i.ctx);
+      hip_check(hipMemcpyAsync(i.dev, i.host, i.bytes, hipMemcpyHostToDevice, i.ctx->stream),
+                "send");
+      i.in_device = true;
+   });
+}
+
+ocipError ocip_API ocipReadImage(ocipImage Image)
+{
+   return guarded([&] {
+      ocip::Image &i = img(Image);
+      if (!i.host) return;  // device-only image (C++/Buffer.cpp:125-126)
+      set_device(*i.ctx);
+      hip_check(hipMemcpyAsync(i.host, i.dev, i.bytes, hipMemcpyDeviceToHost, i.ctx->stream),
+                "read");
+      hip_check(hipStreamSynchronize(i.ctx->stream), "read sync");
+   });
+}
+
+ocipError ocip_API ocipReleaseImage(ocipImage Image)
+{
+   return guarded([&] {
+      ocip::Image &i = img(Image);
+      if (live(i.ctx)) {
+         (void)hipSetDevice(i.ctx->device);
+         // queued work may still use the memory: wait for the context's stream
+         (void)hipStreamSynchronize(i.ctx->stream);
+      }
+      (void)hipFree(i.dev);
+      i.magic = 0;
+      delete &i;
+   });
+}
+
+ocipError ocip_API ocipReleaseProgram(ocipProgram Program)
+{
+   ocip::Program *p = reinterpret_cast<ocip::Program *>(Program);
+   if (!p || p->magic != kProgramMagic) return CL_INVALID_PROGRAM;
+   return guarded([&] {
+      if (live(p->ctx)) {
+         (void)hipSetDevice(p->ctx->device);
+         (void)hipStreamSynchronize(p->ctx->stream);
+      }
+      p->scratch.release();
+      p->magic = 0;
+      delete p;
+   });
+}
+
+// ---- integral ---------------------------------------------------------------------------
+static ocipError prepare_program(ocipProgram *ProgramPtr, ocipImage Image, int kind)
+{
+   if (!g_current) return CL_INVALID_CONTEXT;
+   return guarded([&] {
+      if (!ProgramPtr) fail(CL_INVALID_VALUE, "null program pointer");
+      ocip::Program *p = new ocip::Program();
+      p->ctx = g_current;
+      p->kind = kind;
+      *ProgramPtr = reinterpret_cast<ocipProgram>(p);
+      if (Image) {
+         ocip::Image &i = img(Image);
+         set_device(*g_current);
+         if (kind == kProgIntegral)
+            p->scratch.ensure(integral_aux_bytes(i.desc.Width, i.desc.Height));
+         else
+            p->scratch.ensure(minmax_partials_bytes(i.desc.Width, i.desc.Height));
+      }
+   });
+}
+
+ocipError ocip_API ocipPrepareIntegral(ocipProgram *ProgramPtr, ocipImage Image)
+{
+   return prepare_program(ProgramPtr, Image, kProgIntegral);
+}
+
+// Integral.cpp:97-141 (dest type switch), then PrepareFor + CheckSameSize (:145-146).
+static ocipError run_integral(ocipProgram Program, ocipImage Source, ocipImage Dest, bool sqr)
+{
+   return guarded([&] {
+      ocip::Program &p = prog(Program, kProgIntegral);
+      ocip::Image &S = img(Source), &D = img(Dest);
+      throw_if(check::integral(S.desc, D.desc), "integral arguments");
+      Context &c = *p.ctx;
+      same_context(c, S); same_context(c, D);
+      set_device(c);
+      send_if_needed(c, S);
+      p.scratch.ensure(integral_aux_bytes(S.desc.Width, S.desc.Height));
+      launch_integral(c.stream, view(S), 0, sqr, D.dev, D.desc.Step, D.desc.Type, p.scratch.ptr);
+      D.in_device = true;
+   });
+}
+
+ocipError ocip_API ocipIntegral(ocipProgram Program, ocipImage Source, ocipImage Dest)
+{
+   return run_integral(Program, Source, Dest, false);
+}
+
+ocipError ocip_API ocipSqrIntegral(ocipProgram Program, ocipImage Source, ocipImage Dest)
+{
+   return run_integral(Program, Source, Dest, true);
+}
+
+// ---- proximity --------------------------------------------------------------------------
+ocipError ocip_API ocipPrepareProximity(ocipImage Image)
+{
+   if (!g_current) return CL_INVALID_CONTEXT;
+   return guarded([&] {
+      ocip::Image &i = img(Image);
+      if (i.desc.Channels == 3) fail(CL_IMAGE_FORMAT_NOT_SUPPORTED, "channels");
+   });
+}
+
+#define OCIP_PROX(name, op)                                                                   \
+   ocipError ocip_API name(ocipImage Source, ocipImage Template, ocipImage Dest)             \
+   {                                                                                          \
+      return guarded([&] { run_proximity(op, Source, Template, Dest, ocipPrecisionF32); });  \
+   }
+
+OCIP_PROX(ocipSqrDistance_Norm, kSSDNorm)
+OCIP_PROX(ocipSqrDistance, kSSD)
+OCIP_PROX(ocipAbsDistance, kSAD)
+OCIP_PROX(ocipCrossCorr, kCC)
+OCIP_PROX(ocipCrossCorr_Norm, kCCNorm)
+
+ocipError ocip_API ocipCrossCorrEx(ocipImage Source, ocipImage Template, ocipImage Dest,
+                                   int Precision)
+{
+   return guarded([&] { run_proximity(kCC, Source, Template, Dest, Precision); });
+}
+
+ocipError ocip_API ocipCrossCorr_NormEx(ocipImage Source, ocipImage Template, ocipImage Dest,
+                                        int Precision)
+{
+   return guarded([&] { run_proximity(kCCNorm, Source, Template, Dest, Precision); });
+}
+
+// ---- statistics -------------------------------------------------------------------------
+ocipError ocip_API ocipPrepareStatistics(ocipProgram *ProgramPtr, ocipImage Image)
+{
+   return prepare_program(ProgramPtr, Image, kProgStatistics);
+}
+
+static ocipError run_minmax(ocipProgram Program, ocipImage Source, double *Result, int *IndexX,
+                            int *IndexY, bool want_max)
+{
+   if (!g_current) return CL_INVALID_CONTEXT;
+   return guarded([&] {
+      ocip::Image &S = img(Source);
+      // Statistics.cpp:246 Check1Channel
+      if (S.desc.Channels != 1) fail(CL_INVALID_VALUE, "only 1 channel images are allowed");
+      if (!Result || !IndexX || !IndexY) fail(CL_INVALID_VALUE, "null result pointer");
+      Context &c = Program ? *prog(Program, kProgStatistics).ctx : current();
+      same_context(c, S);
+      set_device(c);
+      send_if_needed(c, S);
+      DevBuf &buf = Program ? reinterpret_cast<ocip::Program *>(Program)->scratch : c.stats;
+      buf.ensure(minmax_partials_bytes(S.desc.Width, S.desc.Height));
+      launch_minmax(c.stream, want_max, view(S), buf.ptr, Result, IndexX, IndexY);
+   });
+}
+
+ocipError ocip_API ocipMinIndx(ocipProgram Program, ocipImage Source, double *Result,
+                               int *IndexX, int *IndexY)
+{
+   return run_minmax(Program, Source, Result, IndexX, IndexY, false);
+}
+
+ocipError ocip_API ocipMaxIndx(ocipProgram Program, ocipImage Source, double *Result,
+                               int *IndexX, int *IndexY)
+{
+   return run_minmax(Program, Source, Result, IndexX, IndexY, true);
+}
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+// Device helpers shared by the kernels: typed element loads and wavefront scans.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ocip_internal.h"
+
+namespace ocip {
+
+template <int TYPE> struct ElemT;
+template <> struct ElemT<SImage::U8>  { typedef uint8_t  type; };
+template <> struct ElemT<SImage::S8>  { typedef int8_t   type; };
+template <> struct ElemT<SImage::U16> { typedef uint16_t type; };
+template <> struct ElemT<SImage::S16> { typedef int16_t  type; };
+template <> struct ElemT<SImage::U32> { typedef uint32_t type; };
+template <> struct ElemT<SImage::S32> { typedef int32_t  type; };
+template <> struct ElemT<SImage::F32> { typedef float    type; };
+template <> struct ElemT<SImage::F64> { typedef double   type; };
+
+// element i of the row starting at `row` (byte address)
+template <int TYPE>
+__device__ __forceinline__ typename ElemT<TYPE>::type ld(const char *row, int i)
+{
+   return reinterpret_cast<const typename ElemT<TYPE>::type *>(row)[i];
+}
+
+// Dispatch a runtime SImage type to a template parameter.
+#define OCIP_TYPE_SWITCH(t, TT, ...)                                     \
+   switch (t) {                                                          \
+   case SImage::U8:  { constexpr int TT = SImage::U8;  __VA_ARGS__; } break;  \
+   case SImage::S8:  { constexpr int TT = SImage::S8;  __VA_ARGS__; } break;  \
+   case SImage::U16: { constexpr int TT = SImage::U16; __VA_ARGS__; } break;  \
+   case SImage::S16: { constexpr int TT = SImage::S16; __VA_ARGS__; } break;  \
+   case SImage::U32: { constexpr int TT = SImage::U32; __VA_ARGS__; } break;  \
+   case SImage::S32: { constexpr int TT = SImage::S32; __VA_ARGS__; } break;  \
+   case SImage::F32: { constexpr int TT = SImage::F32; __VA_ARGS__; } break;  \
+   case SImage::F64: { constexpr int TT = SImage::F64; __VA_ARGS__; } break;  \
+   default: fail(CL_IMAGE_FORMAT_NOT_SUPPORTED, "type");                \
+   }
+
+// Inclusive scan across the 64 lanes of a wavefront.
+template <class T> __device__ __forceinline__ T wave_inclusive_scan(T v)
+{
+   const int lane = threadIdx.x & 63;
+#pragma unroll
+   for (int d = 1; d < 64; d <<= 1) {
+      T o = __shfl_up(v, d, 64);
+      if (lane >= d) v += o;
+   }
+   return v;
+}
+
+template <class T> __device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+   for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+   return v;
+}
+
+inline void launch_check(const char *what) { hip_check(hipGetLastError(), what); }
+
+}  // namespace ocip

@@ -1,0 +1,159 @@
+// Internal declarations shared by the C-ABI layer (capi.cpp) and the HIP kernels.
+//
+// Object model (reference equivalents in brackets):
+//   Context  [COpenCL, C++/OpenCL.cpp:48-69]  one GPU + one in-order HIP stream, plus the
+//            device workspaces the proximity ops reuse between calls.
+//   Image    [Image/Buffer, C++/Image.cpp:147-165, C++/Buffer.cpp:100-194]  Height*Step
+//            bytes of device memory, the caller's host pointer (not copied), an in-device flag.
+//   Program  [Integral / Statistics programs, include/c++/Programs/Integral.h:32-61]
+//            scratch buffers owned by an ocipProgram handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+
+#include "OpenCLIPP.h"
+
+#ifndef CL_INVALID_BUFFER_SIZE
+#define CL_INVALID_BUFFER_SIZE -61
+#endif
+
+namespace ocip {
+
+// An error raised anywhere below the C ABI; the ABI layer returns `code`.
+struct Error {
+   cl_int code;
+   const char *what;
+};
+
+[[noreturn]] inline void fail(cl_int code, const char *what) { throw Error{code, what}; }
+
+inline void hip_check(hipError_t e, const char *what)
+{
+   if (e != hipSuccess) fail(e == hipErrorOutOfMemory ? CL_MEM_OBJECT_ALLOCATION_FAILURE
+                                                      : CL_OUT_OF_RESOURCES, what);
+}
+
+// Growable device buffer (contents are not preserved by `ensure`).
+struct DevBuf {
+   void *ptr = nullptr;
+   size_t bytes = 0;
+   void ensure(size_t n);
+   void release();
+   template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+struct Context;
+
+enum : uint32_t { kImageMagic = 0x474d494fu /*OIMG*/, kContextMagic = 0x5854434fu /*OCTX*/,
+                  kProgramMagic = 0x4752504fu /*OPRG*/ };
+
+struct Image {
+   uint32_t magic = kImageMagic;
+   Context *ctx = nullptr;
+   SImage desc{};
+   void *host = nullptr;
+   void *dev = nullptr;
+   size_t bytes = 0;
+   bool in_device = false;
+};
+
+enum ProgramKind { kProgIntegral = 1, kProgStatistics = 2 };
+
+struct Program {
+   uint32_t magic = kProgramMagic;
+   Context *ctx = nullptr;
+   int kind = 0;
+   DevBuf scratch;
+};
+
+// Per-template data the proximity kernels read: the template as F32 (or F64 / int32)
+// with zero-padded rows, plus FP64 sum of squares.
+struct TemplateBuf {
+   DevBuf data;      // th rows x pitch elements
+   DevBuf sums;      // [0] = sum t^2 (double), [1] = sum t (double)
+};
+
+struct Context {
+   uint32_t magic = kContextMagic;
+   int device = 0;
+   hipStream_t stream = nullptr;
+   std::string name;
+   int num_cus = 0;
+   TemplateBuf tmpl;      // template staging for the single-template ops
+   DevBuf integ;          // FP64 exclusive sum-of-squares integral, (W+1) x (H+1)
+   DevBuf integ_aux;      // integral scan aggregates
+   DevBuf stats;          // reduction partials for MinIndx / MaxIndx
+};
+
+// Kernel timing: construct before a launch, destroy after it.  Records a HIP event pair on
+// `s` when timing is enabled (ocipEnableKernelTiming), nothing otherwise.
+struct KernelTimer {
+   KernelTimer(hipStream_t s, const char *name);
+   ~KernelTimer();
+   hipStream_t stream;
+   const char *name;
+   void *start = nullptr;
+};
+
+// ---------------------------------------------------------------------------------------
+// Element types (SImage::EDataType)
+inline int elem_size(int t)
+{
+   switch (t) {
+   case SImage::U8: case SImage::S8: return 1;
+   case SImage::U16: case SImage::S16: return 2;
+   case SImage::U32: case SImage::S32: case SImage::F32: return 4;
+   case SImage::F64: return 8;
+   default: return 0;
+   }
+}
+inline bool is_float(int t) { return t == SImage::F32 || t == SImage::F64; }
+inline bool is_unsigned(int t) { return t == SImage::U8 || t == SImage::U16 || t == SImage::U32; }
+
+// Proximity operations
+enum ProxOp { kSSD = 0, kSSDNorm = 1, kSAD = 2, kCC = 3, kCCNorm = 4 };
+
+// A device image view handed to kernel launchers.
+struct View {
+   const void *ptr;
+   int width, height, step;   // step in bytes
+   int channels, type;
+};
+
+// ---------------------------------------------------------------------------------------
+// Kernel launchers (defined in the .hip files).  All enqueue on `s` and return immediately.
+
+// Integral of f(src) with f = identity or square, FP64 sums.
+//   out_type F32/F64 with inclusive same-size layout (public API), or
+//   out_type = -1: exclusive padded layout (W+1) x (H+1) doubles (internal, for window sums).
+size_t integral_aux_bytes(int width, int height);
+void launch_integral(hipStream_t s, const View &src, int channel, bool sqr, void *out,
+                     int out_step, int out_type, void *aux);
+
+// Template staging: converts the template to the kernels' compute type with a row pitch of
+// `pitch` elements (zero padding), and writes FP64 sum(t^2), sum(t) to sums[0..1].
+// kind 0: float, 1: double, 2: int32.
+void launch_template_prep(hipStream_t s, const View &tpl, int channel, int pitch, int kind,
+                          void *out, double *sums);
+
+// Plain (VALU) proximity for any type/channel count.  `integ` is the exclusive FP64
+// sum-of-squares integral of the source for the Norm ops (null for the others).
+void launch_proximity(hipStream_t s, int op, const View &src, const View &tpl, void *dst,
+                      int dst_step, const void *tstage, int tpitch, const double *tsums,
+                      const double *integ, int channel);
+
+// MFMA cross-correlation (F32 1-channel source and template), precision 1..3.
+void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &src,
+                           int tw, int th, const float *tstage, int tpitch, const double *tsums,
+                           const double *integ, float *dst, int dst_step, void *work);
+size_t crosscorr_mfma_work_bytes(int precision, int width, int height, int tw, int th);
+
+// Min / max with first-index (row-major) tie break.
+size_t minmax_partials_bytes(int width, int height);
+void launch_minmax(hipStream_t s, bool want_max, const View &src, void *partials,
+                   double *host_result, int *ix, int *iy);
+
+}  // namespace ocip

@@ -1,0 +1,427 @@
+// proximity.hip -- direct (VALU) template matching for every type and channel count.
+//
+// Semantics (reference cl-files/ImageProximity.cl:44-466):
+//   D(x,y) = sum_{v<th} sum_{u<tw} g(s(x+u-tw/2, y+v-th/2), t(u,v)),  s = 0 outside the image,
+//   g = (s-t)^2 (SqrDistance), |s-t| (AbsDistance), s*t (CrossCorr);
+//   the Norm ops divide by sqrt(sum s^2 * sum t^2) over the same zero-padded window.
+//
+// Kernels:
+//   prox_tile_f32  REAL = float sources (F32, U32, S32).  Each workgroup owns a 64-row x
+//                  (4*R)-column output tile; the source tile plus halo is staged in LDS
+//                  (zero padded, converted to float), one band of template rows at a time.
+//                  A thread owns R horizontally adjacent outputs of one row (lanes = rows,
+//                  waves = column groups), keeps a sliding window of R+U-1 source values in
+//                  registers and takes template values as wave-uniform scalar loads, so every
+//                  LDS float feeds up to U FMAs and every template float R FMAs.  Each output
+//                  accumulates in the reference order (template row outer, column inner) with
+//                  one fused multiply-add per tap, which is the reference's order under
+//                  OpenCL's default FP contraction.
+//   prox_tile_int  8- and 16-bit integer sources: the same tiling with exact integer sums
+//                  (int32 where the bound allows, int64 otherwise), rounded once to float.
+//   prox_direct    one output per thread, reference order, any type (F64 sources and
+//                  templates too wide for the LDS tile).
+// Norm denominators come from the FP64 sum-of-squares integral of the source (exclusive,
+// (W+1) x (H+1) layout) by four corners and the FP64 sum of squares of the template.
+#include "device_common.h"
+
+namespace ocip {
+
+namespace {
+
+constexpr int kTileRows = 64;      // output rows per workgroup (one per lane)
+constexpr int kWaves = 4;          // waves per workgroup, side by side in x
+constexpr int kLdsBudget = 64 * 1024;
+
+// ------------------------------------------------------------------------- template staging
+template <int TYPE, int KIND>
+__global__ __launch_bounds__(256) void template_prep(const char *tpl, int tw, int th, int tstep,
+                                                      int nch, int ch, int pitch, void *out,
+                                                      double *sums)
+{
+   __shared__ double red[2][4];
+   double s2 = 0, s1 = 0;
+   const int n = pitch * th;
+   for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const int v = i / pitch, u = i - v * pitch;
+      double val = 0;
+      if (u < tw) val = (double)ld<TYPE>(tpl + (size_t)v * tstep, u * nch + ch);
+      if (KIND == 0) static_cast<float *>(out)[i] = (float)val;
+      else if (KIND == 1) static_cast<double *>(out)[i] = val;
+      else static_cast<int32_t *>(out)[i] = (int32_t)val;
+      s2 += val * val;
+      s1 += val;
+   }
+   s2 = wave_sum(s2);
+   s1 = wave_sum(s1);
+   const int wave = threadIdx.x >> 6;
+   if ((threadIdx.x & 63) == 0) { red[0][wave] = s2; red[1][wave] = s1; }
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      sums[0] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+      sums[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+   }
+}
+
+// window sum of s^2 for output (x, y) from the exclusive padded integral
+__device__ __forceinline__ double window_sqsum(const double *__restrict__ I, int W, int H,
+                                               int x, int y, int tw, int th)
+{
+   const int L = min(max(x - tw / 2, 0), W), R = min(max(x - tw / 2 + tw, 0), W);
+   const int T = min(max(y - th / 2, 0), H), B = min(max(y - th / 2 + th, 0), H);
+   const size_t P = (size_t)W + 1;
+   return (I[B * P + R] - I[T * P + R]) - (I[B * P + L] - I[T * P + L]);
+}
+
+// ------------------------------------------------------------------------- per-tap maths
+template <int OP> __device__ __forceinline__ float tap(float acc, float s, float t)
+{
+   if (OP == kSSD || OP == kSSDNorm) { float d = s - t; return __builtin_fmaf(d, d, acc); }
+   if (OP == kSAD) return acc + __builtin_fabsf(s - t);
+   return __builtin_fmaf(s, t, acc);
+}
+
+// U template columns starting at LDS column `col` (row pointer `lrow`) for R outputs.
+// ALIGNED: col is a multiple of 4 floats, so the window is read with 16-byte LDS loads.
+template <int OP, int R, int U, bool ALIGNED>
+__device__ __forceinline__ void chunk_f32(float (&acc)[R], const float *lrow, int col,
+                                          const float *__restrict__ trow)
+{
+   constexpr int N = R + U - 1;
+   float s[(N + 3) / 4 * 4];
+   if (ALIGNED) {
+#pragma unroll
+      for (int i = 0; i < (N + 3) / 4; ++i) {
+         float4 q = *reinterpret_cast<const float4 *>(lrow + col + 4 * i);
+         s[4 * i] = q.x; s[4 * i + 1] = q.y; s[4 * i + 2] = q.z; s[4 * i + 3] = q.w;
+      }
+   } else {
+#pragma unroll
+      for (int i = 0; i < N; ++i) s[i] = lrow[col + i];
+   }
+   float t[U];
+#pragma unroll
+   for (int k = 0; k < U; ++k) t[k] = trow[k];
+#pragma unroll
+   for (int k = 0; k < U; ++k)
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] = tap<OP>(acc[r], s[r + k], t[k]);
+}
+
+template <int TYPE>
+__device__ __forceinline__ void stage_tile_f32(float *lds, int ldsw, int nrows, int ncols,
+                                               const char *src, int W, int H, int sstep,
+                                               int nch, int ch, int sx0, int sy0)
+{
+   const int n = nrows * ldsw;
+   for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const int r = i / ldsw, c = i - r * ldsw;
+      const int sx = sx0 + c, sy = sy0 + r;
+      float v = 0.f;
+      if (c < ncols && sx >= 0 && sx < W && sy >= 0 && sy < H)
+         v = (float)ld<TYPE>(src + (size_t)sy * sstep, sx * nch + ch);
+      lds[i] = v;
+   }
+}
+
+template <int OP, int TYPE, int R>
+__global__ __launch_bounds__(256) void prox_tile_f32(const char *src, int W, int H, int sstep,
+                                                      int nch, int ch,
+                                                      const float *__restrict__ tp, int tpitch,
+                                                      int tw, int th,
+                                                      const double *__restrict__ tsums,
+                                                      const double *__restrict__ integ,
+                                                      char *dst, int dstep, int vb, int ldsw)
+{
+   extern __shared__ float lds[];
+   constexpr int U = 8;
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   const int x0 = blockIdx.x * (kWaves * R), y0 = blockIdx.y * kTileRows;
+   const int cx = tw / 2, cy = th / 2;
+   const int ncols = kWaves * R + tw - 1;
+   const int nfull = tw / U, rem = tw - nfull * U;
+   const int colbase = wave * R;
+
+   float acc[R];
+#pragma unroll
+   for (int r = 0; r < R; ++r) acc[r] = 0.f;
+
+   for (int v0 = 0; v0 < th; v0 += vb) {
+      const int nv = min(vb, th - v0);
+      __syncthreads();
+      stage_tile_f32<TYPE>(lds, ldsw, kTileRows + nv - 1, ncols, src, W, H, sstep, nch, ch,
+                           x0 - cx, y0 - cy + v0);
+      __syncthreads();
+      for (int v = 0; v < nv; ++v) {
+         const float *lrow = lds + (size_t)(lane + v) * ldsw;
+         const float *trow = tp + (size_t)(v0 + v) * tpitch;
+         int u0 = 0;
+         for (int c = 0; c < nfull; ++c, u0 += U)
+            chunk_f32<OP, R, U, true>(acc, lrow, colbase + u0, trow + u0);
+         if (rem & 4) { chunk_f32<OP, R, 4, true>(acc, lrow, colbase + u0, trow + u0); u0 += 4; }
+         if (rem & 2) { chunk_f32<OP, R, 2, false>(acc, lrow, colbase + u0, trow + u0); u0 += 2; }
+         if (rem & 1) { chunk_f32<OP, R, 1, false>(acc, lrow, colbase + u0, trow + u0); }
+      }
+   }
+
+   const int oy = y0 + lane;
+   if (oy >= H) return;
+   float *drow = reinterpret_cast<float *>(dst + (size_t)oy * dstep);
+   const double tt = (OP == kSSDNorm || OP == kCCNorm) ? tsums[0] : 0.0;
+#pragma unroll
+   for (int r = 0; r < R; ++r) {
+      const int ox = x0 + colbase + r;
+      if (ox < W) {
+         float val = acc[r];
+         if (OP == kSSDNorm || OP == kCCNorm)
+            val = (float)((double)acc[r] / sqrt(window_sqsum(integ, W, H, ox, oy, tw, th) * tt));
+         drow[ox * nch + ch] = val;
+      }
+   }
+}
+
+// ------------------------------------------------------------------------- exact integers
+template <int TYPE>
+__device__ __forceinline__ void stage_tile_int(int32_t *lds, int ldsw, int nrows, int ncols,
+                                               const char *src, int W, int H, int sstep,
+                                               int nch, int ch, int sx0, int sy0)
+{
+   const int n = nrows * ldsw;
+   for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const int r = i / ldsw, c = i - r * ldsw;
+      const int sx = sx0 + c, sy = sy0 + r;
+      int32_t v = 0;
+      if (c < ncols && sx >= 0 && sx < W && sy >= 0 && sy < H)
+         v = (int32_t)ld<TYPE>(src + (size_t)sy * sstep, sx * nch + ch);
+      lds[i] = v;
+   }
+}
+
+template <int OP, class ACC> __device__ __forceinline__ ACC itap(ACC acc, int32_t s, int32_t t)
+{
+   if (OP == kSSD || OP == kSSDNorm) { int32_t d = s - t; return acc + (ACC)d * (ACC)d; }
+   if (OP == kSAD) { int32_t d = s - t; return acc + (ACC)(d < 0 ? -d : d); }
+   return acc + (ACC)s * (ACC)t;
+}
+
+template <int OP, int TYPE, int R, class ACC>
+__global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int H, int sstep,
+                                                      int nch, int ch,
+                                                      const int32_t *__restrict__ tp, int tpitch,
+                                                      int tw, int th,
+                                                      const double *__restrict__ tsums,
+                                                      const double *__restrict__ integ,
+                                                      char *dst, int dstep, int vb, int ldsw)
+{
+   extern __shared__ int32_t ldsi[];
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   const int x0 = blockIdx.x * (kWaves * R), y0 = blockIdx.y * kTileRows;
+   const int cx = tw / 2, cy = th / 2;
+   const int ncols = kWaves * R + tw - 1;
+   const int colbase = wave * R;
+
+   ACC acc[R];
+#pragma unroll
+   for (int r = 0; r < R; ++r) acc[r] = 0;
+
+   for (int v0 = 0; v0 < th; v0 += vb) {
+      const int nv = min(vb, th - v0);
+      __syncthreads();
+      stage_tile_int<TYPE>(ldsi, ldsw, kTileRows + nv - 1, ncols, src, W, H, sstep, nch, ch,
+                           x0 - cx, y0 - cy + v0);
+      __syncthreads();
+      for (int v = 0; v < nv; ++v) {
+         const int32_t *lrow = ldsi + (size_t)(lane + v) * ldsw + colbase;
+         const int32_t *trow = tp + (size_t)(v0 + v) * tpitch;
+         // sliding window over the template row, one column at a time
+         int32_t s[R];
+#pragma unroll
+         for (int r = 0; r < R; ++r) s[r] = lrow[r];
+         for (int u = 0; u < tw; ++u) {
+            const int32_t t = trow[u];
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = itap<OP, ACC>(acc[r], s[r], t);
+#pragma unroll
+            for (int r = 0; r < R - 1; ++r) s[r] = s[r + 1];
+            s[R - 1] = lrow[R + u];
+         }
+      }
+   }
+
+   const int oy = y0 + lane;
+   if (oy >= H) return;
+   float *drow = reinterpret_cast<float *>(dst + (size_t)oy * dstep);
+   const double tt = (OP == kSSDNorm || OP == kCCNorm) ? tsums[0] : 0.0;
+#pragma unroll
+   for (int r = 0; r < R; ++r) {
+      const int ox = x0 + colbase + r;
+      if (ox < W) {
+         float val;
+         if (OP == kSSDNorm || OP == kCCNorm)
+            val = (float)((double)acc[r] / sqrt(window_sqsum(integ, W, H, ox, oy, tw, th) * tt));
+         else
+            val = (float)acc[r];
+         drow[ox * nch + ch] = val;
+      }
+   }
+}
+
+// ------------------------------------------------------------------------- direct
+template <int OP, int TYPE, class REALT, class STAGED>
+__global__ __launch_bounds__(256) void prox_direct(const char *src, int W, int H, int sstep,
+                                                    int nch, int ch,
+                                                    const STAGED *__restrict__ tp, int tpitch,
+                                                    int tw, int th,
+                                                    const double *__restrict__ tsums,
+                                                    const double *__restrict__ integ,
+                                                    char *dst, int dstep)
+{
+   const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+   const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+   if (x >= W || y >= H) return;
+   const int left = -(tw / 2), top = -(th / 2);
+   REALT sum = 0;
+   for (int h = 0; h < th; ++h) {
+      const int py = y + h + top;
+      const bool rowin = py >= 0 && py < H;
+      const char *srow = src + (size_t)(rowin ? py : 0) * sstep;
+      for (int w = 0; w < tw; ++w) {
+         const int px = x + w + left;
+         const REALT t = (REALT)tp[(size_t)h * tpitch + w];
+         REALT s = 0;
+         if (rowin && px >= 0 && px < W) s = (REALT)ld<TYPE>(srow, px * nch + ch);
+         if (OP == kSSD || OP == kSSDNorm) { REALT d = s - t; sum = fma(d, d, sum); }
+         else if (OP == kSAD) sum += fabs(s - t);
+         else sum = fma(s, t, sum);
+      }
+   }
+   REALT val = sum;
+   if (OP == kSSDNorm || OP == kCCNorm)
+      val = (REALT)((double)sum / sqrt(window_sqsum(integ, W, H, x, y, tw, th) * tsums[0]));
+   reinterpret_cast<REALT *>(dst + (size_t)y * dstep)[x * nch + ch] = val;
+}
+
+int lds_pitch(int ncols)
+{
+   // multiple of 4 floats (16-byte loads) and = 4 mod 32 so that lanes reading the same
+   // column of consecutive rows start on different banks
+   int p = (ncols + 4 + 3) / 4 * 4;
+   while (p % 32 != 4) p += 4;
+   return p;
+}
+
+}  // namespace
+
+void launch_template_prep(hipStream_t s, const View &t, int channel, int pitch, int kind,
+                          void *out, double *sums)
+{
+   KernelTimer kt(s, "template_prep");
+   OCIP_TYPE_SWITCH(t.type, TT, {
+      const char *p = static_cast<const char *>(t.ptr);
+      if (kind == 0)
+         template_prep<TT, 0><<<1, 256, 0, s>>>(p, t.width, t.height, t.step, t.channels,
+                                                channel, pitch, out, sums);
+      else if (kind == 1)
+         template_prep<TT, 1><<<1, 256, 0, s>>>(p, t.width, t.height, t.step, t.channels,
+                                                channel, pitch, out, sums);
+      else
+         template_prep<TT, 2><<<1, 256, 0, s>>>(p, t.width, t.height, t.step, t.channels,
+                                                channel, pitch, out, sums);
+   });
+   launch_check("template_prep");
+}
+
+template <int OP>
+static void launch_op(hipStream_t s, const View &S, const View &T, void *dst, int dstep,
+                      const void *tstage, int tpitch, const double *tsums, const double *integ,
+                      int ch)
+{
+   const int W = S.width, H = S.height, tw = T.width, th = T.height;
+   const char *src = static_cast<const char *>(S.ptr);
+   const bool small_int = S.type == SImage::U8 || S.type == SImage::S8 ||
+                          S.type == SImage::U16 || S.type == SImage::S16;
+   const int R = (W >= 2048 || tw * th >= 1024) ? 16 : 8;
+   const int ncols = kWaves * R + tw - 1;
+   const int ldsw = lds_pitch(ncols);
+   const int rows_fit = kLdsBudget / (ldsw * 4) - (kTileRows - 1);
+   const bool tiled = S.type != SImage::F64 && rows_fit >= 1 && th > 0 && tw > 0;
+
+   if (!tiled) {
+      KernelTimer kt(s, "prox_direct");
+      dim3 grid((W + 15) / 16, (H + 15) / 16);
+      if (S.type == SImage::F64) {
+         prox_direct<OP, SImage::F64, double, double><<<grid, 256, 0, s>>>(
+             src, W, H, S.step, S.channels, ch, static_cast<const double *>(tstage), tpitch, tw,
+             th, tsums, integ, static_cast<char *>(dst), dstep);
+      } else if (small_int) {
+         OCIP_TYPE_SWITCH(S.type, TT, {
+            prox_direct<OP, TT, float, int32_t><<<grid, 256, 0, s>>>(
+                src, W, H, S.step, S.channels, ch, static_cast<const int32_t *>(tstage), tpitch,
+                tw, th, tsums, integ, static_cast<char *>(dst), dstep);
+         });
+      } else {
+         OCIP_TYPE_SWITCH(S.type, TT, {
+            prox_direct<OP, TT, float, float><<<grid, 256, 0, s>>>(
+                src, W, H, S.step, S.channels, ch, static_cast<const float *>(tstage), tpitch,
+                tw, th, tsums, integ, static_cast<char *>(dst), dstep);
+         });
+      }
+      launch_check("prox_direct");
+      return;
+   }
+
+   const int vb = min(th, rows_fit);
+   const size_t lds = (size_t)(kTileRows + vb - 1) * ldsw * 4;
+   dim3 grid((W + kWaves * R - 1) / (kWaves * R), (H + kTileRows - 1) / kTileRows);
+
+#define OCIP_TILE_F32(TT_, R_)                                                                 \
+   prox_tile_f32<OP, TT_, R_><<<grid, 256, lds, s>>>(                                         \
+       src, W, H, S.step, S.channels, ch, static_cast<const float *>(tstage), tpitch, tw, th, \
+       tsums, integ, static_cast<char *>(dst), dstep, vb, ldsw)
+#define OCIP_TILE_INT(TT_, R_, ACC_)                                                             \
+   prox_tile_int<OP, TT_, R_, ACC_><<<grid, 256, lds, s>>>(                                     \
+       src, W, H, S.step, S.channels, ch, static_cast<const int32_t *>(tstage), tpitch, tw, th, \
+       tsums, integ, static_cast<char *>(dst), dstep, vb, ldsw)
+
+   if (small_int) {
+      KernelTimer kt(s, "prox_tile_int");
+      // int32 accumulation is exact when taps * max|term| < 2^31 (8-bit sources,
+      // templates up to 33025 taps); 16-bit sources accumulate in int64.
+      const bool bits8 = S.type == SImage::U8 || S.type == SImage::S8;
+      const bool fits32 = bits8 && (long long)tw * th * 65025LL < (1LL << 31);
+      OCIP_TYPE_SWITCH(S.type, TT, {
+         if (fits32) {
+            if (R == 16) OCIP_TILE_INT(TT, 16, int32_t); else OCIP_TILE_INT(TT, 8, int32_t);
+         } else {
+            if (R == 16) OCIP_TILE_INT(TT, 16, long long); else OCIP_TILE_INT(TT, 8, long long);
+         }
+      });
+      launch_check("prox_tile_int");
+   } else {
+      KernelTimer kt(s, "prox_tile_f32");
+      switch (S.type) {
+      case SImage::F32: if (R == 16) OCIP_TILE_F32(SImage::F32, 16); else OCIP_TILE_F32(SImage::F32, 8); break;
+      case SImage::U32: if (R == 16) OCIP_TILE_F32(SImage::U32, 16); else OCIP_TILE_F32(SImage::U32, 8); break;
+      case SImage::S32: if (R == 16) OCIP_TILE_F32(SImage::S32, 16); else OCIP_TILE_F32(SImage::S32, 8); break;
+      default: fail(CL_IMAGE_FORMAT_NOT_SUPPORTED, "type");
+      }
+      launch_check("prox_tile_f32");
+   }
+#undef OCIP_TILE_F32
+#undef OCIP_TILE_INT
+}
+
+void launch_proximity(hipStream_t s, int op, const View &S, const View &T, void *dst, int dstep,
+                      const void *tstage, int tpitch, const double *tsums, const double *integ,
+                      int ch)
+{
+   switch (op) {
+   case kSSD: launch_op<kSSD>(s, S, T, dst, dstep, tstage, tpitch, tsums, integ, ch); break;
+   case kSSDNorm: launch_op<kSSDNorm>(s, S, T, dst, dstep, tstage, tpitch, tsums, integ, ch); break;
+   case kSAD: launch_op<kSAD>(s, S, T, dst, dstep, tstage, tpitch, tsums, integ, ch); break;
+   case kCC: launch_op<kCC>(s, S, T, dst, dstep, tstage, tpitch, tsums, integ, ch); break;
+   case kCCNorm: launch_op<kCCNorm>(s, S, T, dst, dstep, tstage, tpitch, tsums, integ, ch); break;
+   default: fail(CL_INVALID_VALUE, "op");
+   }
+}
+
+}  // namespace ocip

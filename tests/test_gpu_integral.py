@@ -1,0 +1,154 @@
+"""GPU parity of ocipIntegral / ocipSqrIntegral, ocipMinIndx / ocipMaxIndx and the ABI's
+error paths, through the C ABI.
+
+Integral bar: 8/16-bit integer sources are bit-exact against the exact sums (F64 dest) and
+against the exact sums rounded once to float (F32 dest); float sources are within 1e-13
+relative of the exact (long double) sums for F64 dest and within one float ulp for F32.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.gpu_util import say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ocip():
+    from openclipp_amd import ocip as mod
+    ctx = mod.initialize()
+    yield mod
+    mod.lib.ocipUninitialize(ctx)
+
+
+def run_integral(ocip, src, sqr, dtype):
+    S = ocip.Image(src)
+    D = ocip.Image(shape=src.shape, dtype=dtype)
+    prog = ctypes.c_void_p()
+    ocip.check(ocip.lib.ocipPrepareIntegral(ctypes.byref(prog), S.handle), "prepare")
+    fn = ocip.lib.ocipSqrIntegral if sqr else ocip.lib.ocipIntegral
+    ocip.check(fn(prog, S.handle, D.handle), "integral")
+    out = D.read().copy()
+    ocip.check(ocip.lib.ocipReleaseProgram(prog), "release")
+    return out
+
+
+SHAPES = [(1, 1), (5, 7), (8, 64), (37, 1023), (130, 1025), (64, 4096), (300, 2049)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("dtype", [np.uint8, np.int16])
+@pytest.mark.parametrize("sqr", [False, True])
+def test_integral_integer_exact(ocip, shape, dtype, sqr):
+    rng = np.random.default_rng(shape[0] * 7 + shape[1])
+    info = np.iinfo(dtype)
+    src = rng.integers(info.min, info.max, size=shape, endpoint=True).astype(dtype)
+    exact = orc.integral_exact(src, sqr)
+    say("integral", shape, np.dtype(dtype).name, sqr)
+    np.testing.assert_array_equal(run_integral(ocip, src, sqr, np.float64), exact)
+    np.testing.assert_array_equal(run_integral(ocip, src, sqr, np.float32), exact.astype(np.float32))
+
+
+@pytest.mark.parametrize("shape", [(33, 70), (257, 1100)])
+@pytest.mark.parametrize("sqr", [False, True])
+def test_integral_float(ocip, shape, sqr):
+    rng = np.random.default_rng(1)
+    src = rng.random(shape).astype(np.float32)
+    exact = orc.integral_exact(src, sqr)
+    np.testing.assert_allclose(run_integral(ocip, src, sqr, np.float64), exact, rtol=1e-13)
+    got32 = run_integral(ocip, src, sqr, np.float32).astype(np.float64)
+    assert np.all(np.abs(got32 - exact) <= np.spacing(exact.astype(np.float32)).astype(np.float64))
+
+
+def test_integral_4096_u8_sqr_exact(ocip):
+    from openclipp_amd import workloads
+    src, _, _ = workloads.integer_4096()
+    say("integral 4096 u8")
+    got = run_integral(ocip, src, True, np.float64)
+    a = src.astype(np.int64)
+    want = np.cumsum(np.cumsum(a * a, axis=0), axis=1).astype(np.float64)
+    np.testing.assert_array_equal(got, want)
+    # invariant: the last element is the total
+    assert got[-1, -1] == float((a * a).sum())
+
+
+def test_integral_argument_errors(ocip):
+    src = np.zeros((8, 8), np.uint8)
+    S = ocip.Image(src)
+    prog = ctypes.c_void_p()
+    ocip.check(ocip.lib.ocipPrepareIntegral(ctypes.byref(prog), S.handle))
+    bad_type = ocip.Image(shape=(8, 8), dtype=np.int32)
+    bad_size = ocip.Image(shape=(8, 9), dtype=np.float32)
+    assert ocip.lib.ocipIntegral(prog, S.handle, bad_type.handle) == ocip.CL_INVALID_VALUE
+    assert ocip.lib.ocipIntegral(prog, S.handle, bad_size.handle) == ocip.CL_INVALID_IMAGE_SIZE
+    assert ocip.lib.ocipIntegral(None, S.handle, bad_size.handle) == ocip.CL_INVALID_PROGRAM
+    ocip.check(ocip.lib.ocipReleaseProgram(prog))
+
+
+# ----------------------------------------------------------------------------- statistics
+@pytest.mark.parametrize("dtype", [np.float32, np.uint8, np.int16, np.float64])
+def test_min_max_indx(ocip, dtype):
+    rng = np.random.default_rng(9)
+    if np.issubdtype(dtype, np.integer):
+        src = rng.integers(0, 100, size=(300, 517)).astype(dtype)
+    else:
+        src = rng.random((300, 517)).astype(dtype)
+    S = ocip.Image(src)
+    v, x, y = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+    for want_max in (True, False):
+        fn = ocip.lib.ocipMaxIndx if want_max else ocip.lib.ocipMinIndx
+        ocip.check(fn(None, S.handle, ctypes.byref(v), ctypes.byref(x), ctypes.byref(y)))
+        assert (v.value, x.value, y.value) == orc.min_max_indx(src, want_max)
+
+
+# ----------------------------------------------------------------------------- ABI behaviour
+def test_context_and_images(ocip):
+    name = ctypes.create_string_buffer(256)
+    ocip.check(ocip.lib.ocipGetDeviceName(name, 256))
+    assert b"gfx950" in name.value
+    assert ocip.lib.ocipGetDeviceName(ctypes.create_string_buffer(2), 2) == ocip.CL_OUT_OF_RESOURCES
+    ocip.check(ocip.lib.ocipFinish())
+    # device-only images: Send and Read are no-ops
+    D = ocip.Image(shape=(4, 4), dtype=np.float32, device_only=True)
+    ocip.check(ocip.lib.ocipSendImage(D.handle))
+    ocip.check(ocip.lib.ocipReadImage(D.handle))
+    h = ctypes.c_void_p()
+    assert ocip.lib.ocipCreateImage(ctypes.byref(h), ocip.SImage(4, 4, 16, 3, ocip.F32), None,
+                                    ocip.CL_MEM_READ_WRITE) == ocip.CL_IMAGE_FORMAT_NOT_SUPPORTED
+
+
+def test_proximity_argument_errors(ocip):
+    S = ocip.Image(np.zeros((32, 32), np.float32))
+    T = ocip.Image(np.zeros((4, 4), np.float32))
+    big = ocip.Image(np.zeros((40, 4), np.float32))
+    T8 = ocip.Image(np.zeros((4, 4), np.uint8))
+    D = ocip.Image(np.zeros((32, 32), np.float32))
+    D64 = ocip.Image(np.zeros((32, 32), np.float64))
+    Dsmall = ocip.Image(np.zeros((31, 32), np.float32))
+    L = ocip.lib
+    assert L.ocipCrossCorr(S.handle, big.handle, D.handle) == ocip.CL_INVALID_VALUE
+    assert L.ocipCrossCorr(S.handle, T8.handle, D.handle) == ocip.CL_INVALID_VALUE
+    assert L.ocipCrossCorr(S.handle, T.handle, Dsmall.handle) == ocip.CL_INVALID_IMAGE_SIZE
+    assert L.ocipCrossCorr(S.handle, T.handle, D64.handle) == ocip.CL_INVALID_VALUE
+    assert L.ocipCrossCorr(S.handle, T.handle, None) == ocip.CL_INVALID_MEM_OBJECT
+    ocip.check(L.ocipPrepareProximity(S.handle))
+
+
+def test_send_is_lazy_and_explicit(ocip):
+    """A source never sent is sent on first use; later host changes need ocipSendImage."""
+    from openclipp_amd import ocip as mod
+    src = np.ones((16, 16), np.float32)
+    tpl = np.ones((1, 1), np.float32)
+    S, T = mod.Image(src), mod.Image(tpl)
+    D = mod.Image(shape=(16, 16), dtype=np.float32)
+    mod.check(mod.proximity("CrossCorr", S, T, D))
+    assert D.read()[3, 3] == 1.0
+    src[:] = 2.0
+    mod.check(mod.proximity("CrossCorr", S, T, D))
+    assert D.read()[3, 3] == 1.0          # device copy unchanged
+    S.send()
+    mod.check(mod.proximity("CrossCorr", S, T, D))
+    assert D.read()[3, 3] == 2.0

@@ -1,0 +1,222 @@
+"""GPU parity of the five proximity ops through the C ABI against the CPU oracle.
+
+Bars:
+* F32 / U32 / S32 / F64 sources, SqrDistance / AbsDistance / CrossCorr: bit-exact against
+  the reference-order restatement with fused multiply-add (cl-files/ImageProximity.cl:55-70
+  under OpenCL's default FP contraction);
+* 8/16-bit integer sources, SqrDistance / AbsDistance / CrossCorr: bit-exact against the
+  exact int64 sums rounded to float;
+* the Norm ops (denominator from the FP64 sum-of-squares integral): within 1e-5 relative
+  of the reference-order map (plus a 1e-6 absolute floor), and within the float32
+  accumulation bound of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.gpu_util import OP_CODE, band_oracle, run_op, say
+
+pytestmark = pytest.mark.gpu
+
+RAW_OPS = ["SqrDistance", "AbsDistance", "CrossCorr"]
+NORM_OPS = ["SqrDistance_Norm", "CrossCorr_Norm"]
+SIZES = [(37, 29), (64, 64), (130, 70)]
+TEMPLATES = [(1, 1), (2, 3), (5, 5), (8, 8), (16, 16), (17, 15), (31, 31), (33, 7), (12, 40)]
+
+
+@pytest.fixture(scope="module")
+def ocip():
+    from openclipp_amd import ocip as mod
+    ctx = mod.initialize()
+    yield mod
+    mod.lib.ocipUninitialize(ctx)
+
+
+def rand(rng, shape, dtype):
+    if np.issubdtype(dtype, np.integer):
+        info = np.iinfo(dtype)
+        return rng.integers(info.min, info.max, size=shape, endpoint=True).astype(dtype)
+    return rng.random(shape).astype(dtype)
+
+
+def norm_close(got, ref, f64, n):
+    got = got.astype(np.float64)
+    ref = ref.astype(np.float64)
+    assert np.all(np.abs(got - ref) <= 1e-5 * np.abs(ref) + 1e-6), np.max(np.abs(got - ref))
+    assert np.all(np.abs(got - f64) <= 3 * (n + 4) * 2.0 ** -24 * np.abs(f64) + 1e-6)
+
+
+@pytest.mark.parametrize("W,H", SIZES)
+@pytest.mark.parametrize("tw,th", TEMPLATES)
+def test_f32_raw_bit_exact(ocip, W, H, tw, th):
+    if tw > W or th > H:
+        pytest.skip("template larger than source")
+    rng = np.random.default_rng(W * 1000 + H * 10 + tw * 7 + th)
+    src = rand(rng, (H, W), np.float32)
+    tpl = rand(rng, (th, tw), np.float32)
+    for op in RAW_OPS:
+        say("f32", op, W, H, tw, th)
+        got = run_op(ocip, op, src, tpl)
+        want = orc.prox_ref(OP_CODE[op], src, tpl, contract=True)
+        np.testing.assert_array_equal(got, want, err_msg=op)
+
+
+@pytest.mark.parametrize("W,H", SIZES)
+@pytest.mark.parametrize("tw,th", [(1, 1), (4, 4), (16, 16), (17, 15), (31, 31)])
+def test_f32_norm(ocip, W, H, tw, th):
+    rng = np.random.default_rng(5 + W + tw)
+    src = rand(rng, (H, W), np.float32)
+    tpl = rand(rng, (th, tw), np.float32)
+    for op in NORM_OPS:
+        say("f32", op, W, H, tw, th)
+        got = run_op(ocip, op, src, tpl)
+        norm_close(got, orc.prox_ref(OP_CODE[op], src, tpl), orc.prox_f64(OP_CODE[op], src, tpl),
+                   tw * th)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int8, np.uint16, np.int16])
+@pytest.mark.parametrize("tw,th", [(3, 3), (16, 16), (32, 32), (9, 20)])
+def test_integer_exact(ocip, dtype, tw, th):
+    rng = np.random.default_rng(tw * th)
+    src = rand(rng, (70, 90), dtype)
+    tpl = rand(rng, (th, tw), dtype)
+    for op in RAW_OPS:
+        say(np.dtype(dtype).name, op, tw, th)
+        got = run_op(ocip, op, src, tpl)
+        want = orc.f64_from_i64(OP_CODE[op], src, tpl).astype(np.float32)
+        np.testing.assert_array_equal(got, want, err_msg=op)
+    for op in NORM_OPS:
+        got = run_op(ocip, op, src, tpl).astype(np.float64)
+        want = orc.f64_from_i64(OP_CODE[op], src, tpl)
+        np.testing.assert_allclose(got, want, rtol=2.0 ** -23, atol=0)
+
+
+@pytest.mark.parametrize("dtype", [np.uint32, np.int32])
+def test_32bit_integer_reference_order(ocip, dtype):
+    """REAL is float for 32-bit integer sources (cl-files/Buffers.h:58-63): bit-exact
+    against the reference-order float restatement."""
+    rng = np.random.default_rng(77)
+    src = rng.integers(0, 5000, size=(50, 61)).astype(dtype)
+    tpl = rng.integers(0, 5000, size=(9, 7)).astype(dtype)
+    for op in RAW_OPS:
+        got = run_op(ocip, op, src, tpl)
+        np.testing.assert_array_equal(got, orc.prox_ref(OP_CODE[op], src, tpl), err_msg=op)
+
+
+def test_f64_source(ocip):
+    rng = np.random.default_rng(64)
+    src = rng.random((41, 43))
+    tpl = rng.random((7, 6))
+    for op in RAW_OPS:
+        got = run_op(ocip, op, src, tpl)
+        assert got.dtype == np.float64
+        np.testing.assert_array_equal(got, orc.prox_ref(OP_CODE[op], src, tpl), err_msg=op)
+    for op in NORM_OPS:
+        got = run_op(ocip, op, src, tpl)
+        np.testing.assert_allclose(got, orc.prox_f64(OP_CODE[op], src, tpl), rtol=1e-12)
+
+
+@pytest.mark.parametrize("nch", [2, 4])
+@pytest.mark.parametrize("dtype", [np.float32, np.uint8])
+def test_multichannel(ocip, nch, dtype):
+    rng = np.random.default_rng(nch)
+    src = rand(rng, (33, 40, nch), dtype)
+    tpl = rand(rng, (6, 5, nch), dtype)
+    for op in RAW_OPS:
+        got = run_op(ocip, op, src, tpl)
+        if dtype == np.uint8:
+            want = orc.f64_from_i64(OP_CODE[op], src, tpl).astype(np.float32)
+        else:
+            want = orc.prox_ref(OP_CODE[op], src, tpl)
+        np.testing.assert_array_equal(got, want, err_msg=op)
+    for op in NORM_OPS:
+        got = run_op(ocip, op, src, tpl).astype(np.float64)
+        np.testing.assert_allclose(got, orc.prox_f64(OP_CODE[op], src, tpl), rtol=1e-5, atol=1e-6)
+
+
+def test_pitched_images(ocip):
+    """Source with padding at the end of each row, template that is an ROI of the source
+    (the Bench's template, Bench/src/benchImageProximity.hpp:83-86)."""
+    rng = np.random.default_rng(3)
+    big = rng.random((60, 80), dtype=np.float32)
+    src = big[:, :70]                    # pitch 80 floats, width 70
+    tpl = big[10:26, 10:26]              # 16x16 ROI with the source's pitch
+    assert src.strides[0] == 320 and tpl.strides[0] == 320
+    for op in RAW_OPS:
+        got = run_op(ocip, op, src, tpl)
+        np.testing.assert_array_equal(got, orc.prox_ref(OP_CODE[op], src, tpl), err_msg=op)
+
+
+def test_template_too_wide_for_lds_tile(ocip):
+    """Templates wider than the LDS tile budget take the direct kernel (same order)."""
+    rng = np.random.default_rng(300)
+    src = rng.random((24, 320), dtype=np.float32)
+    tpl = rng.random((3, 300), dtype=np.float32)
+    for op in RAW_OPS:
+        got = run_op(ocip, op, src, tpl)
+        np.testing.assert_array_equal(got, orc.prox_ref(OP_CODE[op], src, tpl), err_msg=op)
+
+
+def test_template_equal_to_source(ocip):
+    rng = np.random.default_rng(4)
+    src = rng.random((21, 18), dtype=np.float32)
+    got = run_op(ocip, "CrossCorr", src, src.copy())
+    np.testing.assert_array_equal(got, orc.prox_ref(orc.CC, src, src.copy()))
+
+
+def test_nan_on_empty_window(ocip):
+    src = np.zeros((20, 20), np.float32)
+    src[10, 10] = 1
+    tpl = np.ones((3, 3), np.float32)
+    got = run_op(ocip, "CrossCorr_Norm", src, tpl)
+    assert np.isnan(got[0, 0]) and got[10, 10] == np.float32(1 / 3)
+
+
+# ------------------------------------------------------------------------ full-size workloads
+def test_plain_1024_invariants(ocip):
+    from openclipp_amd import workloads
+    src, tpl, (ox, oy) = workloads.plain_1024()
+    say("plain_1024")
+    ncc = run_op(ocip, "CrossCorr_Norm", src, tpl)
+    ssd = run_op(ocip, "SqrDistance", src, tpl)
+    y, x = oy + 8, ox + 8
+    assert np.unravel_index(np.nanargmax(ncc), ncc.shape) == (y, x)
+    assert np.unravel_index(np.argmin(ssd), ssd.shape) == (y, x)
+    assert np.nanmax(ncc) <= 1.0 + 1e-6
+    for y0, y1 in [(0, 8), (500, 508), (1016, 1024), (y - 2, y + 3)]:
+        want = band_oracle(lambda c: orc.prox_ref(orc.SSD, c, tpl), src, tpl, y0, y1)
+        np.testing.assert_array_equal(ssd[y0:y1], want)
+        want = band_oracle(lambda c: orc.prox_ref(orc.CC_NORM, c, tpl), src, tpl, y0, y1)
+        assert np.all(np.abs(ncc[y0:y1] - want) <= 1e-5 * np.abs(want) + 1e-6)
+
+
+def test_integer_4096_bands(ocip):
+    from openclipp_amd import workloads
+    src, tpl, (ox, oy) = workloads.integer_4096()
+    y, x = oy + 16, ox + 16
+    for op in RAW_OPS + NORM_OPS:
+        say("integer_4096", op)
+        got = run_op(ocip, op, src, tpl)
+        for y0, y1 in [(0, 4), (2046, 2050), (4092, 4096), (y - 1, y + 2)]:
+            want = band_oracle(lambda c: orc.f64_from_i64(OP_CODE[op], c, tpl), src, tpl, y0, y1)
+            if op in RAW_OPS:
+                np.testing.assert_array_equal(got[y0:y1], want.astype(np.float32), err_msg=op)
+            else:
+                np.testing.assert_allclose(got[y0:y1], want, rtol=2.0 ** -23, atol=0, err_msg=op)
+        if op == "SqrDistance":
+            # exact value at the crop: the sum of squared noise
+            noise = tpl.astype(np.int64) - src[oy:oy + 32, ox:ox + 32].astype(np.int64)
+            assert got[y, x] == np.float32((noise * noise).sum())
+
+
+def test_precision_4096_plain_bands(ocip):
+    from openclipp_amd import workloads
+    src, tpl, (ox, oy) = workloads.precision_4096()
+    say("precision_4096 plain")
+    cc = run_op(ocip, "CrossCorr", src, tpl)
+    ncc = run_op(ocip, "CrossCorr_Norm", src, tpl)
+    y, x = oy + 32, ox + 32
+    assert abs(ncc[y, x] - 1.0) < 1e-5
+    for y0, y1 in [(0, 2), (2047, 2049), (4094, 4096)]:
+        want = band_oracle(lambda c: orc.prox_ref(orc.CC, c, tpl, nthreads=8), src, tpl, y0, y1)
+        np.testing.assert_array_equal(cc[y0:y1], want)
