@@ -7,8 +7,9 @@ Bars:
 * 8/16-bit integer sources, SqrDistance / AbsDistance / CrossCorr: bit-exact against the
   exact int64 sums rounded to float;
 * the Norm ops (denominator from the FP64 sum-of-squares integral): within 1e-5 relative
-  of the reference-order map (plus a 1e-6 absolute floor), and within the float32
-  accumulation bound of the FP64 oracle.
+  of the reference-order map (plus a 1e-6 absolute floor) wherever the window holds at
+  least 1e-6 of the image's energy, and everywhere within the float32 accumulation bound
+  plus the integral's conditioning term of the FP64 oracle (see norm_close).
 """
 import numpy as np
 import pytest
@@ -39,11 +40,26 @@ def rand(rng, shape, dtype):
     return rng.random(shape).astype(dtype)
 
 
-def norm_close(got, ref, f64, n):
+def norm_close(got, ref, f64, n, src=None, tpl=None):
+    """Norm bar.  The denominator's window sum of s^2 is a 4-corner difference of the FP64
+    integral, whose absolute error is ~eps64 * (total image energy); its relative error is
+    therefore eps64 * kappa with kappa = total energy / window energy.  Where the window
+    holds at least 1e-6 of the image energy (kappa <= 1e6) the bar is 1e-5 relative to the
+    reference-order map; elsewhere the map is ill-conditioned and is held to the FP64 oracle
+    with the conditioning term included."""
     got = got.astype(np.float64)
     ref = ref.astype(np.float64)
-    assert np.all(np.abs(got - ref) <= 1e-5 * np.abs(ref) + 1e-6), np.max(np.abs(got - ref))
-    assert np.all(np.abs(got - f64) <= 3 * (n + 4) * 2.0 ** -24 * np.abs(f64) + 1e-6)
+    kappa = np.ones_like(f64)
+    if src is not None:
+        s = src.astype(np.float64)
+        ss = orc.prox_f64(orc.CC, s * s, np.ones(tpl.shape[:2], np.float64))
+        with np.errstate(divide="ignore"):
+            kappa = (s * s).sum() / ss
+    well = kappa <= 1e6
+    d = np.abs(got - ref)
+    assert np.all(d[well] <= 1e-5 * np.abs(ref[well]) + 1e-6), np.max(d[well])
+    bound = (3 * (n + 4) * 2.0 ** -24 + 16 * 2.0 ** -53 * kappa) * np.abs(f64) + 1e-6
+    assert np.all(np.abs(got - f64) <= bound)
 
 
 @pytest.mark.parametrize("W,H", SIZES)
@@ -64,6 +80,8 @@ def test_f32_raw_bit_exact(ocip, W, H, tw, th):
 @pytest.mark.parametrize("W,H", SIZES)
 @pytest.mark.parametrize("tw,th", [(1, 1), (4, 4), (16, 16), (17, 15), (31, 31)])
 def test_f32_norm(ocip, W, H, tw, th):
+    if tw > W or th > H:
+        pytest.skip("template larger than source")
     rng = np.random.default_rng(5 + W + tw)
     src = rand(rng, (H, W), np.float32)
     tpl = rand(rng, (th, tw), np.float32)
@@ -71,7 +89,7 @@ def test_f32_norm(ocip, W, H, tw, th):
         say("f32", op, W, H, tw, th)
         got = run_op(ocip, op, src, tpl)
         norm_close(got, orc.prox_ref(OP_CODE[op], src, tpl), orc.prox_f64(OP_CODE[op], src, tpl),
-                   tw * th)
+                   tw * th, src, tpl)
 
 
 @pytest.mark.parametrize("dtype", [np.uint8, np.int8, np.uint16, np.int16])
