@@ -158,6 +158,53 @@ void run_proximity(int op, ocipImage hs, ocipImage ht, ocipImage hd, int precisi
    D.in_device = true;
 }
 
+// Gradients of L = sum(G * op(S, T)) (extensions; F32 1-channel images).
+void run_backward(int mode, ocipImage hs, ocipImage ht, ocipImage hg, ocipImage hds,
+                  ocipImage hdt)
+{
+   Context &c = current();
+   Image &S = img(hs), &T = img(ht), &G = img(hg);
+   Image *dS = hds ? &img(hds) : nullptr;
+   Image *dT = hdt ? &img(hdt) : nullptr;
+   auto f32_1 = [](const Image &i) {
+      return i.desc.Type == SImage::F32 && i.desc.Channels == 1;
+   };
+   if (T.desc.Width > S.desc.Width || T.desc.Height > S.desc.Height)
+      fail(CL_INVALID_VALUE, "template larger than source");
+   if (!f32_1(S) || !f32_1(T) || !f32_1(G)) fail(CL_INVALID_VALUE, "F32 1-channel images");
+   check_same_size(S, G);
+   if (dS) {
+      if (!f32_1(*dS)) fail(CL_INVALID_VALUE, "F32 1-channel images");
+      check_same_size(S, *dS);
+   }
+   if (dT) {
+      if (!f32_1(*dT)) fail(CL_INVALID_VALUE, "F32 1-channel images");
+      check_same_size(T, *dT);
+   }
+   if (T.desc.Width == 0 || T.desc.Height == 0) fail(CL_INVALID_VALUE, "empty template");
+   same_context(c, S); same_context(c, T); same_context(c, G);
+   if (dS) same_context(c, *dS);
+   if (dT) same_context(c, *dT);
+   set_device(c);
+   send_if_needed(c, S);
+   send_if_needed(c, T);
+   send_if_needed(c, G);
+   const int tw = T.desc.Width, th = T.desc.Height, W = S.desc.Width, H = S.desc.Height;
+   const int tpitch = tpitch_for(tw);
+   c.tmpl.data.ensure((size_t)tpitch * th * sizeof(float));
+   c.tmpl.sums.ensure(2 * sizeof(double));
+   const double *integ = mode == 0 ? sqr_integral_for(c, S, 0) : nullptr;
+   c.integ_aux.ensure(integral_aux_bytes(W, H));
+   launch_template_prep(c.stream, view(T), 0, tpitch, 0, c.tmpl.data.ptr, c.tmpl.sums.as<double>());
+   c.work.ensure(backward_work_bytes(W, H, tw, th, tpitch));
+   launch_backward(c.stream, mode, view(S), view(G), c.tmpl.data.as<float>(), tpitch, tw, th,
+                   c.tmpl.sums.as<double>(), integ, c.work.ptr, c.integ_aux.ptr,
+                   dS ? static_cast<char *>(dS->dev) : nullptr, dS ? (int)dS->desc.Step : 0,
+                   dT ? static_cast<char *>(dT->dev) : nullptr, dT ? (int)dT->desc.Step : 0);
+   if (dS) dS->in_device = true;
+   if (dT) dT->in_device = true;
+}
+
 }  // namespace
 
 void DevBuf::ensure(size_t n)
@@ -230,6 +277,7 @@ ocipError ocip_API ocipUninitialize(ocipContext Context)
       c->integ.release();
       c->integ_aux.release();
       c->stats.release();
+      c->work.release();
       (void)hipStreamDestroy(c->stream);
       c->magic = 0;
       delete c;
@@ -449,6 +497,20 @@ ocipError ocip_API ocipCrossCorr_NormEx(ocipImage Source, ocipImage Template, oc
                                         int Precision)
 {
    return guarded([&] { run_proximity(kCCNorm, Source, Template, Dest, Precision); });
+}
+
+ocipError ocip_API ocipCrossCorr_NormBackward(ocipImage Source, ocipImage Template,
+                                              ocipImage GradDest, ocipImage GradSource,
+                                              ocipImage GradTemplate)
+{
+   return guarded([&] { run_backward(0, Source, Template, GradDest, GradSource, GradTemplate); });
+}
+
+ocipError ocip_API ocipSqrDistanceBackward(ocipImage Source, ocipImage Template,
+                                           ocipImage GradDest, ocipImage GradSource,
+                                           ocipImage GradTemplate)
+{
+   return guarded([&] { run_backward(1, Source, Template, GradDest, GradSource, GradTemplate); });
 }
 
 // ---- statistics -------------------------------------------------------------------------

@@ -60,4 +60,15 @@ template <class T> __device__ __forceinline__ T wave_sum(T v)
 
 inline void launch_check(const char *what) { hip_check(hipGetLastError(), what); }
 
+// Sum over columns [xa, xb) and rows [ya, yb), clamped to the image, from an exclusive
+// (W+1) x (H+1) FP64 integral (I[j][i] = sum over rows < j and columns < i).
+__device__ __forceinline__ double rect_sum(const double *__restrict__ I, int W, int H, int xa,
+                                           int xb, int ya, int yb)
+{
+   xa = min(max(xa, 0), W); xb = min(max(xb, 0), W);
+   ya = min(max(ya, 0), H); yb = min(max(yb, 0), H);
+   const size_t P = (size_t)W + 1;
+   return (I[yb * P + xb] - I[ya * P + xb]) - (I[yb * P + xa] - I[ya * P + xa]);
+}
+
 }  // namespace ocip

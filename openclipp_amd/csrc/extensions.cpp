@@ -6,8 +6,4 @@ ocipError ocip_API ocipCrossCorr_NormBatch(ocipImage, const ocipImage *, ocipIma
 { return CL_INVALID_OPERATION; }
 ocipError ocip_API ocipSqrDistance_NormBatch(ocipImage, const ocipImage *, ocipImage *, uint, int)
 { return CL_INVALID_OPERATION; }
-ocipError ocip_API ocipCrossCorr_NormBackward(ocipImage, ocipImage, ocipImage, ocipImage, ocipImage)
-{ return CL_INVALID_OPERATION; }
-ocipError ocip_API ocipSqrDistanceBackward(ocipImage, ocipImage, ocipImage, ocipImage, ocipImage)
-{ return CL_INVALID_OPERATION; }
 }

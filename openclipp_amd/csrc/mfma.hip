@@ -20,7 +20,8 @@
 // Tiling: a workgroup (4 waves) owns 128 output rows x 128 output columns; wave w owns rows
 // 32w..32w+31 and four 32-column n-tiles.  The source tile plus halo is staged in LDS as
 // 16-bit values (two planes, hi and lo, for split BF16), one band of template rows at a
-// time.  Templates wider than 97 columns are cut into 64-column blocks (KC = 6 each).
+// time.  Templates wider than 97 columns are cut into 64-column blocks (KC = 6 each), staged
+// one block at a time.
 #include "device_common.h"
 
 namespace ocip {
@@ -141,38 +142,39 @@ __global__ __launch_bounds__(256) void crosscorr_mfma(const float *__restrict__ 
 #pragma unroll
       for (int i = 0; i < 16; ++i) acc[j][i] = 0.f;
 
-   for (int v0 = 0; v0 < th; v0 += vb) {
-      const int nv = min(vb, th - v0);
-      const int nrows = kTileRows + nv - 1;
-      const size_t plane = (size_t)nrows * ldsp;
-      __syncthreads();
-      // stage rows [y0 - cy + v0, +nrows) x columns [x0 - cx, +lcols), 8 columns per item
-      const int groups = lcols / 8;
-      for (int it = threadIdx.x; it < nrows * groups; it += 256) {
-         const int r = it / groups, g = it - r * groups;
-         const int sy = y0 - cy + v0 + r, sx = x0 - cx + 8 * g;
-         u16x8 h0, h1;
-         const bool rowin = sy >= 0 && sy < H;
-         const float *srow = reinterpret_cast<const float *>(
-             reinterpret_cast<const char *>(src) + (size_t)(rowin ? sy : 0) * sstep);
+   // template column blocks (one block unless tw > 97), then bands of template rows
+   for (int ub = 0; ub < nub; ++ub) {
+      for (int v0 = 0; v0 < th; v0 += vb) {
+         const int nv = min(vb, th - v0);
+         const int nrows = kTileRows + nv - 1;
+         const size_t plane = (size_t)nrows * ldsp;
+         __syncthreads();
+         // stage rows [y0 - cy + v0, +nrows) x columns [x0 - cx + 64 ub, +lcols), 8 per item
+         const int groups = lcols / 8;
+         for (int it = threadIdx.x; it < nrows * groups; it += 256) {
+            const int r = it / groups, g = it - r * groups;
+            const int sy = y0 - cy + v0 + r, sx = x0 - cx + ub * kBlockW + 8 * g;
+            u16x8 h0, h1;
+            const bool rowin = sy >= 0 && sy < H;
+            const float *srow = reinterpret_cast<const float *>(
+                reinterpret_cast<const char *>(src) + (size_t)(rowin ? sy : 0) * sstep);
 #pragma unroll
-         for (int i = 0; i < 8; ++i) {
-            const int x = sx + i;
-            const float f = (rowin && x >= 0 && x < W) ? srow[x] : 0.f;
-            unsigned short a, b;
-            encode<P>(f, a, b);
-            h0[i] = a;
-            h1[i] = b;
+            for (int i = 0; i < 8; ++i) {
+               const int x = sx + i;
+               const float f = (rowin && x >= 0 && x < W) ? srow[x] : 0.f;
+               unsigned short a, b;
+               encode<P>(f, a, b);
+               h0[i] = a;
+               h1[i] = b;
+            }
+            *reinterpret_cast<u16x8 *>(lds + (size_t)r * ldsp + 8 * g) = h0;
+            if (NPL == 2) *reinterpret_cast<u16x8 *>(lds + plane + (size_t)r * ldsp + 8 * g) = h1;
          }
-         *reinterpret_cast<u16x8 *>(lds + (size_t)r * ldsp + 8 * g) = h0;
-         if (NPL == 2) *reinterpret_cast<u16x8 *>(lds + plane + (size_t)r * ldsp + 8 * g) = h1;
-      }
-      __syncthreads();
+         __syncthreads();
 
-      for (int v = 0; v < nv; ++v) {
-         const unsigned short *arow =
-             lds + (size_t)(wave * 32 + (lane & 31) + v) * ldsp + 8 * (lane >> 5);
-         for (int ub = 0; ub < nub; ++ub) {
+         for (int v = 0; v < nv; ++v) {
+            const unsigned short *ab =
+                lds + (size_t)(wave * 32 + (lane & 31) + v) * ldsp + 8 * (lane >> 5);
             const size_t fb = ((size_t)(ub * th + v0 + v) * nfrag_v) * 64 + lane;
             u16x8 b0[KC], b1[KC];
 #pragma unroll
@@ -180,7 +182,6 @@ __global__ __launch_bounds__(256) void crosscorr_mfma(const float *__restrict__ 
                b0[kc] = frag0[fb + (size_t)kc * 64];
                if (NPL == 2) b1[kc] = frag1[fb + (size_t)kc * 64];
             }
-            const unsigned short *ab = arow + ub * kBlockW;
 #pragma unroll
             for (int c = 0; c < 2 * (kNT - 1) + KC; ++c) {
                const u16x8 a0 = *reinterpret_cast<const u16x8 *>(ab + 16 * c);
@@ -236,8 +237,8 @@ Geometry geometry(int precision, int tw, int th)
       g.KC = 6;
       g.nub = (tw + kBlockW - 1) / kBlockW;
    }
-   // A reads reach column 64*(nub-1) + 16*(2*(kNT-1) + KC - 1) + 15
-   g.lcols = kBlockW * (g.nub - 1) + 16 * (2 * (kNT - 1) + g.KC);
+   // A reads reach column 16*(2*(kNT-1) + KC - 1) + 15 of the staged block
+   g.lcols = 16 * (2 * (kNT - 1) + g.KC);
    // row pitch: 16-byte aligned rows, shifted by 16 bytes per row modulo 128 bytes
    g.ldsp = (g.lcols + 7) / 8 * 8;
    while ((g.ldsp * 2) % 128 != 16) g.ldsp += 8;

@@ -85,7 +85,8 @@ struct Context {
    TemplateBuf tmpl;      // template staging for the single-template ops
    DevBuf integ;          // FP64 exclusive sum-of-squares integral, (W+1) x (H+1)
    DevBuf integ_aux;      // integral scan aggregates
-   DevBuf stats;          // reduction partials for MinIndx / MaxIndx
+   DevBuf stats;          // reduction partials for MinIndx / MaxIndx, MFMA fragments
+   DevBuf work;           // gradient and batch workspaces
 };
 
 // Kernel timing: construct before a launch, destroy after it.  Records a HIP event pair on
@@ -145,11 +146,22 @@ void launch_proximity(hipStream_t s, int op, const View &src, const View &tpl, v
                       int dst_step, const void *tstage, int tpitch, const double *tsums,
                       const double *integ, int channel);
 
+// FP32 cross-correlation of a 1-channel F32 map with a staged float template whose origin
+// sits at (-ax, -ay) (the ops above use ax = tw/2, ay = th/2).
+void launch_crosscorr_f32_anchored(hipStream_t s, const View &M, const float *tstage, int tpitch,
+                                   int tw, int th, int ax, int ay, float *dst, int dstep);
+
 // MFMA cross-correlation (F32 1-channel source and template), precision 1..3.
 void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &src,
                            int tw, int th, const float *tstage, int tpitch, const double *tsums,
                            const double *integ, float *dst, int dst_step, void *work);
 size_t crosscorr_mfma_work_bytes(int precision, int width, int height, int tw, int th);
+
+// Gradients (backward.hip).  mode 0: CrossCorr_Norm, 1: SqrDistance.  dS / dT may be null.
+size_t backward_work_bytes(int W, int H, int tw, int th, int tpitch);
+void launch_backward(hipStream_t s, int mode, const View &S, const View &G, const float *tstage,
+                     int tpitch, int tw, int th, const double *tsums, const double *integ_ss,
+                     void *work, void *aux, char *dS, int dsstep, char *dT, int dtstep);
 
 // Min / max with first-index (row-major) tie break.
 size_t minmax_partials_bytes(int width, int height);

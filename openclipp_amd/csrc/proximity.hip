@@ -130,13 +130,13 @@ __global__ __launch_bounds__(256) void prox_tile_f32(const char *src, int W, int
                                                       int tw, int th,
                                                       const double *__restrict__ tsums,
                                                       const double *__restrict__ integ,
-                                                      char *dst, int dstep, int vb, int ldsw)
+                                                      char *dst, int dstep, int vb, int ldsw,
+                                                      int cx, int cy)
 {
    extern __shared__ float lds[];
    constexpr int U = 8;
    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
    const int x0 = blockIdx.x * (kWaves * R), y0 = blockIdx.y * kTileRows;
-   const int cx = tw / 2, cy = th / 2;
    const int ncols = kWaves * R + tw - 1;
    const int nfull = tw / U, rem = tw - nfull * U;
    const int colbase = wave * R;
@@ -376,7 +376,7 @@ static void launch_op(hipStream_t s, const View &S, const View &T, void *dst, in
 #define OCIP_TILE_F32(TT_, R_)                                                                 \
    prox_tile_f32<OP, TT_, R_><<<grid, 256, lds, s>>>(                                         \
        src, W, H, S.step, S.channels, ch, static_cast<const float *>(tstage), tpitch, tw, th, \
-       tsums, integ, static_cast<char *>(dst), dstep, vb, ldsw)
+       tsums, integ, static_cast<char *>(dst), dstep, vb, ldsw, tw / 2, th / 2)
 #define OCIP_TILE_INT(TT_, R_, ACC_)                                                             \
    prox_tile_int<OP, TT_, R_, ACC_><<<grid, 256, lds, s>>>(                                     \
        src, W, H, S.step, S.channels, ch, static_cast<const int32_t *>(tstage), tpitch, tw, th, \
@@ -408,6 +408,26 @@ static void launch_op(hipStream_t s, const View &S, const View &T, void *dst, in
    }
 #undef OCIP_TILE_F32
 #undef OCIP_TILE_INT
+}
+
+void launch_crosscorr_f32_anchored(hipStream_t s, const View &M, const float *tstage, int tpitch,
+                                   int tw, int th, int ax, int ay, float *dst, int dstep)
+{
+   const int W = M.width, H = M.height;
+   constexpr int R = 16;
+   const int ncols = kWaves * R + tw - 1;
+   const int ldsw = lds_pitch(ncols);
+   const int rows_fit = kLdsBudget / (ldsw * 4) - (kTileRows - 1);
+   if (rows_fit < 1 || M.type != SImage::F32 || M.channels != 1)
+      fail(CL_INVALID_VALUE, "anchored correlation: unsupported shape");
+   const int vb = min(th, rows_fit);
+   const size_t lds = (size_t)(kTileRows + vb - 1) * ldsw * 4;
+   dim3 grid((W + kWaves * R - 1) / (kWaves * R), (H + kTileRows - 1) / kTileRows);
+   KernelTimer kt(s, "prox_tile_f32");
+   prox_tile_f32<kCC, SImage::F32, R><<<grid, 256, lds, s>>>(
+       static_cast<const char *>(M.ptr), W, H, M.step, 1, 0, tstage, tpitch, tw, th, nullptr,
+       nullptr, reinterpret_cast<char *>(dst), dstep, vb, ldsw, ax, ay);
+   launch_check("prox_tile_f32");
 }
 
 void launch_proximity(hipStream_t s, int op, const View &S, const View &T, void *dst, int dstep,

@@ -60,3 +60,12 @@ def band_oracle(fn, src, tpl, y0, y1):
     # need rows >= y0 - cy >= r0, and rows < r1, so the band is exact.
     out = fn(crop)
     return out[y0 - r0:y1 - r0]
+
+
+def integral_kappa(src, tpl):
+    """Conditioning of a Norm denominator taken from the FP64 integral: total image energy
+    over window energy.  The 4-corner window sum has relative error ~eps64 * kappa."""
+    s = src.astype(np.float64)
+    ss = orc.prox_f64(orc.CC, s * s, np.ones(tpl.shape[:2], np.float64))
+    with np.errstate(divide="ignore"):
+        return (s * s).sum() / ss

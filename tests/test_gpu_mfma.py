@@ -1,0 +1,110 @@
+"""GPU checks of the MFMA cross-correlation (ocipCrossCorrEx / ocipCrossCorr_NormEx).
+
+Against the FP64 oracle with bounds that follow from the operand precision:
+  BF16      |err| <= (2^-7  + acc) * sum|s*t|   (8-bit significands: u = 2^-8 per operand)
+  FP16      |err| <= (2^-10 + acc) * sum|s*t|   (11-bit significands: u = 2^-11 per operand)
+  BF16x3    |err| <= (2^-14 + acc) * sum|s*t|   (hi + lo leaves 2^-16 per operand, and the
+                                                 dropped lo*lo term is below 2^-16)
+with acc = 2 (n + 16) 2^-24 for the FP32 accumulation of n products.  Norm maps use the same
+bound divided by the exact denominator.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.gpu_util import integral_kappa, run_op, say
+
+pytestmark = pytest.mark.gpu
+
+PREC = {"bf16": 1, "f16": 2, "bf16x3": 3}
+UNIT = {"bf16": 2.0 ** -7 + 2.0 ** -16, "f16": 2.0 ** -10 + 2.0 ** -22, "bf16x3": 2.0 ** -14}
+
+
+@pytest.fixture(scope="module")
+def ocip():
+    from openclipp_amd import ocip as mod
+    ctx = mod.initialize()
+    yield mod
+    mod.lib.ocipUninitialize(ctx)
+
+
+def bound(prec, src, tpl):
+    n = tpl.size
+    sabs = orc.prox_f64(orc.CC, np.abs(src).astype(np.float64), np.abs(tpl).astype(np.float64))
+    return (UNIT[prec] + 2 * (n + 16) * 2.0 ** -24) * sabs + 1e-30
+
+
+TEMPLATES = [(1, 1), (3, 5), (16, 16), (31, 31), (64, 64), (96, 40), (17, 64), (100, 9), (130, 3)]
+
+
+@pytest.mark.parametrize("prec", list(PREC))
+@pytest.mark.parametrize("tw,th", TEMPLATES)
+def test_crosscorr_bounds(ocip, prec, tw, th):
+    rng = np.random.default_rng(tw * 131 + th)
+    src = rng.random((150, 260), dtype=np.float32) - np.float32(0.3)
+    tpl = rng.random((th, tw), dtype=np.float32) - np.float32(0.3)
+    say("mfma", prec, tw, th)
+    f64 = orc.prox_f64(orc.CC, src, tpl)
+    b = bound(prec, src, tpl)
+    got = run_op(ocip, "CrossCorr", src, tpl, precision=PREC[prec]).astype(np.float64)
+    err = np.abs(got - f64)
+    assert np.all(err <= b), (float(np.max(err / b)), prec)
+    ncc64 = orc.prox_f64(orc.CC_NORM, src, tpl)
+    s64, t64 = src.astype(np.float64), tpl.astype(np.float64)
+    den = np.sqrt(orc.prox_f64(orc.CC, s64 * s64, np.ones_like(t64)) * (t64 * t64).sum())
+    got = run_op(ocip, "CrossCorr_Norm", src, tpl, precision=PREC[prec]).astype(np.float64)
+    ok = np.isfinite(ncc64)
+    cond = 16 * 2.0 ** -53 * integral_kappa(src, tpl) * np.abs(ncc64)
+    assert np.all(np.abs(got - ncc64)[ok] <= (b / den + cond)[ok] * 1.01 + 1e-7)
+
+
+@pytest.mark.parametrize("prec", list(PREC))
+def test_image_edges_and_odd_sizes(ocip, prec):
+    """Image sizes that are not multiples of the 128x128 tile."""
+    rng = np.random.default_rng(2)
+    for H, W in [(1, 1), (7, 300), (129, 129), (257, 65)]:
+        src = rng.random((H, W), dtype=np.float32)
+        tpl = rng.random((min(H, 9), min(W, 12)), dtype=np.float32)
+        f64 = orc.prox_f64(orc.CC, src, tpl)
+        got = run_op(ocip, "CrossCorr", src, tpl, precision=PREC[prec]).astype(np.float64)
+        assert np.all(np.abs(got - f64) <= bound(prec, src, tpl)), (H, W)
+
+
+def test_f32_precision_is_the_plain_path(ocip):
+    rng = np.random.default_rng(5)
+    src = rng.random((70, 90), dtype=np.float32)
+    tpl = rng.random((11, 13), dtype=np.float32)
+    a = run_op(ocip, "CrossCorr", src, tpl, precision=0)
+    np.testing.assert_array_equal(a, orc.prox_ref(orc.CC, src, tpl))
+
+
+def test_precision_argument_errors(ocip):
+    S = ocip.Image(np.zeros((32, 32), np.uint8))
+    T = ocip.Image(np.zeros((4, 4), np.uint8))
+    D = ocip.Image(np.zeros((32, 32), np.float32))
+    assert ocip.lib.ocipCrossCorrEx(S.handle, T.handle, D.handle, 1) == ocip.CL_INVALID_VALUE
+    S2 = ocip.Image(np.zeros((32, 32), np.float32))
+    T2 = ocip.Image(np.zeros((4, 4), np.float32))
+    assert ocip.lib.ocipCrossCorrEx(S2.handle, T2.handle, D.handle, 9) == ocip.CL_INVALID_VALUE
+
+
+@pytest.mark.parametrize("prec,limit", [("bf16", 1e-3), ("f16", 2.5e-4), ("bf16x3", 2e-5)])
+def test_precision_4096_ncc_error(ocip, prec, limit):
+    """The 4096x4096 F32 N(0.5, 0.2) image with a 64x64 crop x1.1: max |NCC - NCC_fp64| over
+    the match location and 4096 sampled pixels, against the stated limits (
+    1e-3 for BF16)."""
+    from openclipp_amd import workloads
+    src, tpl, (ox, oy) = workloads.precision_4096()
+    say("precision_4096", prec)
+    got = run_op(ocip, "CrossCorr_Norm", src, tpl, precision=PREC[prec])
+    rng = np.random.default_rng(0)
+    ys = np.concatenate([rng.integers(0, 4096, 4095), [oy + 32]])
+    xs = np.concatenate([rng.integers(0, 4096, 4095), [ox + 32]])
+    P = np.pad(src.astype(np.float64), [(32, 31), (32, 31)])
+    t = tpl.astype(np.float64)
+    win = np.stack([P[y:y + 64, x:x + 64] for x, y in zip(xs, ys)])
+    want = (win * t).sum(axis=(1, 2)) / np.sqrt((win * win).sum(axis=(1, 2)) * (t * t).sum())
+    err = np.abs(got[ys, xs] - want)
+    say(prec, "max", err.max(), "mean", err.mean())
+    assert err.max() <= limit
+    assert np.unravel_index(np.nanargmax(got), got.shape) == (oy + 32, ox + 32)

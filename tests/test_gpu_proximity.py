@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as orc
-from tests.gpu_util import OP_CODE, band_oracle, run_op, say
+from tests.gpu_util import OP_CODE, band_oracle, integral_kappa, run_op, say
 
 pytestmark = pytest.mark.gpu
 
@@ -49,12 +49,7 @@ def norm_close(got, ref, f64, n, src=None, tpl=None):
     with the conditioning term included."""
     got = got.astype(np.float64)
     ref = ref.astype(np.float64)
-    kappa = np.ones_like(f64)
-    if src is not None:
-        s = src.astype(np.float64)
-        ss = orc.prox_f64(orc.CC, s * s, np.ones(tpl.shape[:2], np.float64))
-        with np.errstate(divide="ignore"):
-            kappa = (s * s).sum() / ss
+    kappa = np.ones_like(f64) if src is None else integral_kappa(src, tpl)
     well = kappa <= 1e6
     d = np.abs(got - ref)
     assert np.all(d[well] <= 1e-5 * np.abs(ref[well]) + 1e-6), np.max(d[well])
