@@ -198,7 +198,7 @@ BwLayout bw_layout(int W, int H, int tw, int th, int tpitch)
    L.F = o;       o = align_up(o + px * sizeof(float));
    L.Gd = o;      o = align_up(o + px * sizeof(float));
    L.E = o;       o = align_up(o + px * sizeof(float));
-   L.I = o;       o = align_up(o + (size_t)(W + 1) * (H + 1) * sizeof(double));
+   L.I = o;       o = align_up(o + excl_bytes(W, H));
    L.partial = o; o = align_up(o + nb * tw * th * sizeof(float));
    L.scal = o;    o = align_up(o + 1024 * sizeof(double));
    L.tflip = o;   o = align_up(o + (size_t)tpitch * th * sizeof(float));
@@ -282,7 +282,7 @@ void launch_backward(hipStream_t s, int mode, const View &S, const View &G, cons
                                     F, W * 4);
       // integral of the map whose flipped window sums multiply s
       View bv = mode == 0 ? View{E, W, H, W * 4, 1, SImage::F32} : G;
-      launch_integral(s, bv, 0, false, I, (W + 1) * (int)sizeof(double), -1, aux);
+      launch_integral(s, bv, 0, false, I, 0, -1, aux);
       const float alpha = mode == 0 ? 1.f : -2.f, beta = mode == 0 ? -1.f : 2.f;
       grad_source<<<grid_for((long long)px), kThreads, 0, s>>>(F, Sp, S.step, I, W, H, tw, th,
                                                                alpha, beta, dS, dsstep);

@@ -100,19 +100,14 @@ void throw_if(cl_int code, const char *what)
 }
 
 // ---- proximity driver ---------------------------------------------------------------------
-size_t integ_bytes(const Image &S)
-{
-   return (size_t)(S.desc.Width + 1) * (S.desc.Height + 1) * sizeof(double);
-}
 
 // FP64 exclusive sum-of-squares integral of channel c of S into c.integ.
 const double *sqr_integral_for(Context &c, const Image &S, int channel)
 {
    View v = view(S);
-   c.integ.ensure(integ_bytes(S));
+   c.integ.ensure(excl_bytes(v.width, v.height));
    c.integ_aux.ensure(integral_aux_bytes(v.width, v.height));
-   launch_integral(c.stream, v, channel, true, c.integ.ptr, (v.width + 1) * (int)sizeof(double),
-                   -1, c.integ_aux.ptr);
+   launch_integral(c.stream, v, channel, true, c.integ.ptr, 0, -1, c.integ_aux.ptr);
    return c.integ.as<double>();
 }
 

@@ -60,15 +60,24 @@ template <class T> __device__ __forceinline__ T wave_sum(T v)
 
 inline void launch_check(const char *what) { hip_check(hipGetLastError(), what); }
 
-// Sum over columns [xa, xb) and rows [ya, yb), clamped to the image, from an exclusive
-// (W+1) x (H+1) FP64 integral (I[j][i] = sum over rows < j and columns < i).
+// Sum over columns [xa, xb) and rows [ya, yb), clamped to the image, from the internal
+// exclusive FP64 integral (layout: excl_pitch / kExclOffset in ocip_internal.h).
 __device__ __forceinline__ double rect_sum(const double *__restrict__ I, int W, int H, int xa,
                                            int xb, int ya, int yb)
 {
    xa = min(max(xa, 0), W); xb = min(max(xb, 0), W);
    ya = min(max(ya, 0), H); yb = min(max(yb, 0), H);
-   const size_t P = (size_t)W + 1;
-   return (I[yb * P + xb] - I[ya * P + xb]) - (I[yb * P + xa] - I[ya * P + xa]);
+   const size_t P = excl_pitch(W);
+   const double *r0 = I + ya * P + kExclOffset, *r1 = I + yb * P + kExclOffset;
+   return (r1[xb] - r0[xb]) - (r1[xa] - r0[xa]);
+}
+
+// Window sum of the Norm denominators: the template window of output (x, y) with origin at
+// (-tw/2, -th/2) (cl-files/ImageProximityFFT.cl:55-84 gives the same window).
+__device__ __forceinline__ double window_sqsum(const double *__restrict__ I, int W, int H,
+                                               int x, int y, int tw, int th)
+{
+   return rect_sum(I, W, H, x - tw / 2, x - tw / 2 + tw, y - th / 2, y - th / 2 + th);
 }
 
 }  // namespace ocip

@@ -107,26 +107,55 @@ __device__ __forceinline__ f32x16 mma(u16x8 a, u16x8 b, f32x16 c)
                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
-__device__ __forceinline__ double window_sqsum(const double *__restrict__ I, int W, int H,
-                                               int x, int y, int tw, int th)
+
+template <int NPL, int KC>
+__device__ __forceinline__ void load_frags(const u16x8 *__restrict__ f0, const u16x8 *__restrict__ f1,
+                                           size_t base, u16x8 (&b0)[KC], u16x8 (&b1)[KC])
 {
-   const int L = min(max(x - tw / 2, 0), W), R = min(max(x - tw / 2 + tw, 0), W);
-   const int T = min(max(y - th / 2, 0), H), B = min(max(y - th / 2 + th, 0), H);
-   const size_t Pp = (size_t)W + 1;
-   return (I[B * Pp + R] - I[T * Pp + R]) - (I[B * Pp + L] - I[T * Pp + L]);
+#pragma unroll
+   for (int kc = 0; kc < KC; ++kc) {
+      b0[kc] = f0[base + (size_t)kc * 64];
+      if (NPL == 2) b1[kc] = f1[base + (size_t)kc * 64];
+   }
+}
+
+// One template row: A fragments from the LDS row `ab` (16 columns apart), KC B fragments.
+template <int P, int NPL, int KC>
+__device__ __forceinline__ void row_mfma(f32x16 (&acc)[kNT], const unsigned short *ab,
+                                         size_t plane, const u16x8 (&b0)[KC],
+                                         const u16x8 (&b1)[KC])
+{
+#pragma unroll
+   for (int c = 0; c < 2 * (kNT - 1) + KC; ++c) {
+      const u16x8 a0 = *reinterpret_cast<const u16x8 *>(ab + 16 * c);
+      u16x8 a1;
+      if (NPL == 2) a1 = *reinterpret_cast<const u16x8 *>(ab + plane + 16 * c);
+#pragma unroll
+      for (int j = 0; j < kNT; ++j) {
+         const int kc = c - 2 * j;
+         if (kc < 0 || kc >= KC) continue;
+         acc[j] = mma<P>(a0, b0[kc], acc[j]);
+         if (NPL == 2) {
+            acc[j] = mma<P>(a0, b1[kc], acc[j]);
+            acc[j] = mma<P>(a1, b0[kc], acc[j]);
+         }
+      }
+   }
 }
 
 // ldsp: LDS row pitch in 16-bit elements (multiple of 8); lcols: staged columns (multiple
 // of 8); vb: template rows per LDS band.
+// LDS use limits the kernel to two workgroups per CU (two waves per SIMD), so the compiler
+// may spend registers on keeping two template rows of B fragments in flight.
 template <int P, int KC, bool NORM>
-__global__ __launch_bounds__(256) void crosscorr_mfma(const float *__restrict__ src, int W,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void crosscorr_mfma(const float *__restrict__ src, int W,
                                                       int H, int sstep, int tw, int th, int nub,
                                                       const u16x8 *__restrict__ frag0,
                                                       const u16x8 *__restrict__ frag1,
                                                       const double *__restrict__ tsums,
                                                       const double *__restrict__ integ,
                                                       float *dst, int dstep, int vb, int ldsp,
-                                                      int lcols)
+                                                      int lcols, bool vec)
 {
    extern __shared__ u16x8 lds_v[];
    unsigned short *lds = reinterpret_cast<unsigned short *>(lds_v);
@@ -134,7 +163,6 @@ __global__ __launch_bounds__(256) void crosscorr_mfma(const float *__restrict__ 
    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
    const int x0 = blockIdx.x * kTileCols, y0 = blockIdx.y * kTileRows;
    const int cx = tw / 2, cy = th / 2;
-   const int nfrag_v = KC;  // fragments per (ub, v)
 
    f32x16 acc[kNT];
 #pragma unroll
@@ -149,56 +177,69 @@ __global__ __launch_bounds__(256) void crosscorr_mfma(const float *__restrict__ 
          const int nrows = kTileRows + nv - 1;
          const size_t plane = (size_t)nrows * ldsp;
          __syncthreads();
-         // stage rows [y0 - cy + v0, +nrows) x columns [x0 - cx + 64 ub, +lcols), 8 per item
+         // stage rows [y0 - cy + v0, +nrows) x columns [x0 - cx + 64 ub, +lcols), 8 per item,
+         // four items per thread in flight
          const int groups = lcols / 8;
-         for (int it = threadIdx.x; it < nrows * groups; it += 256) {
-            const int r = it / groups, g = it - r * groups;
-            const int sy = y0 - cy + v0 + r, sx = x0 - cx + ub * kBlockW + 8 * g;
-            u16x8 h0, h1;
-            const bool rowin = sy >= 0 && sy < H;
-            const float *srow = reinterpret_cast<const float *>(
-                reinterpret_cast<const char *>(src) + (size_t)(rowin ? sy : 0) * sstep);
+         const int total = nrows * groups;
+         for (int it0 = threadIdx.x; it0 < total; it0 += 256 * 4) {
+            float f[4][8];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-               const int x = sx + i;
-               const float f = (rowin && x >= 0 && x < W) ? srow[x] : 0.f;
-               unsigned short a, b;
-               encode<P>(f, a, b);
-               h0[i] = a;
-               h1[i] = b;
-            }
-            *reinterpret_cast<u16x8 *>(lds + (size_t)r * ldsp + 8 * g) = h0;
-            if (NPL == 2) *reinterpret_cast<u16x8 *>(lds + plane + (size_t)r * ldsp + 8 * g) = h1;
-         }
-         __syncthreads();
-
-         for (int v = 0; v < nv; ++v) {
-            const unsigned short *ab =
-                lds + (size_t)(wave * 32 + (lane & 31) + v) * ldsp + 8 * (lane >> 5);
-            const size_t fb = ((size_t)(ub * th + v0 + v) * nfrag_v) * 64 + lane;
-            u16x8 b0[KC], b1[KC];
+            for (int k = 0; k < 4; ++k) {
+               const int it = it0 + 256 * k;
+               const int r = it / groups, g = it - r * groups;
+               const int sy = y0 - cy + v0 + r, sx = x0 - cx + ub * kBlockW + 8 * g;
+               const bool rowin = it < total && sy >= 0 && sy < H;
+               const float *srow = reinterpret_cast<const float *>(
+                   reinterpret_cast<const char *>(src) + (size_t)(rowin ? sy : 0) * sstep);
+               if (vec && rowin && sx >= 0 && sx + 8 <= W) {
+                  const float4 q0 = *reinterpret_cast<const float4 *>(srow + sx);
+                  const float4 q1 = *reinterpret_cast<const float4 *>(srow + sx + 4);
+                  f[k][0] = q0.x; f[k][1] = q0.y; f[k][2] = q0.z; f[k][3] = q0.w;
+                  f[k][4] = q1.x; f[k][5] = q1.y; f[k][6] = q1.z; f[k][7] = q1.w;
+               } else {
 #pragma unroll
-            for (int kc = 0; kc < KC; ++kc) {
-               b0[kc] = frag0[fb + (size_t)kc * 64];
-               if (NPL == 2) b1[kc] = frag1[fb + (size_t)kc * 64];
-            }
-#pragma unroll
-            for (int c = 0; c < 2 * (kNT - 1) + KC; ++c) {
-               const u16x8 a0 = *reinterpret_cast<const u16x8 *>(ab + 16 * c);
-               u16x8 a1;
-               if (NPL == 2) a1 = *reinterpret_cast<const u16x8 *>(ab + plane + 16 * c);
-#pragma unroll
-               for (int j = 0; j < kNT; ++j) {
-                  const int kc = c - 2 * j;
-                  if (kc < 0 || kc >= KC) continue;
-                  acc[j] = mma<P>(a0, b0[kc], acc[j]);
-                  if (NPL == 2) {
-                     acc[j] = mma<P>(a0, b1[kc], acc[j]);
-                     acc[j] = mma<P>(a1, b0[kc], acc[j]);
+                  for (int i = 0; i < 8; ++i) {
+                     const int x = sx + i;
+                     f[k][i] = (rowin && x >= 0 && x < W) ? srow[x] : 0.f;
                   }
                }
             }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+               const int it = it0 + 256 * k;
+               if (it >= total) break;
+               const int r = it / groups, g = it - r * groups;
+               u16x8 h0, h1;
+#pragma unroll
+               for (int i = 0; i < 8; ++i) {
+                  unsigned short a, b;
+                  encode<P>(f[k][i], a, b);
+                  h0[i] = a;
+                  h1[i] = b;
+               }
+               *reinterpret_cast<u16x8 *>(lds + (size_t)r * ldsp + 8 * g) = h0;
+               if (NPL == 2) *reinterpret_cast<u16x8 *>(lds + plane + (size_t)r * ldsp + 8 * g) = h1;
+            }
          }
+         __syncthreads();
+
+         // ping-pong: the B fragments of template row v+1 load while row v computes
+         const size_t fb0 = ((size_t)(ub * th + v0) * KC) * 64 + lane;
+         u16x8 pa0[KC], pa1[KC], pb0[KC], pb1[KC];
+         load_frags<NPL, KC>(frag0, frag1, fb0, pa0, pa1);
+         const unsigned short *arow = lds + (size_t)(wave * 32 + (lane & 31)) * ldsp + 8 * (lane >> 5);
+         int v = 0;
+         for (; v + 1 < nv; v += 2) {
+            // keep each prefetch ahead of the MFMAs of the row before it
+            load_frags<NPL, KC>(frag0, frag1, fb0 + (size_t)(v + 1) * KC * 64, pb0, pb1);
+            __builtin_amdgcn_sched_barrier(0);
+            row_mfma<P, NPL, KC>(acc, arow + (size_t)v * ldsp, plane, pa0, pa1);
+            const int vn = v + 2 < nv ? v + 2 : v + 1;
+            load_frags<NPL, KC>(frag0, frag1, fb0 + (size_t)vn * KC * 64, pa0, pa1);
+            __builtin_amdgcn_sched_barrier(0);
+            row_mfma<P, NPL, KC>(acc, arow + (size_t)(v + 1) * ldsp, plane, pb0, pb1);
+         }
+         if (v < nv) row_mfma<P, NPL, KC>(acc, arow + (size_t)v * ldsp, plane, pa0, pa1);
       }
    }
 
@@ -255,13 +296,17 @@ void launch_main(hipStream_t s, const View &S, int tw, int th, const Geometry &g
                  float *dst, int dstep)
 {
    auto kern = crosscorr_mfma<P, KC, NORM>;
+   // staged columns start at x0 - cx + 64 ub + 8 g with x0 a multiple of 128: 16-byte
+   // aligned when cx is a multiple of 4 and rows are 16-byte aligned
+   const bool vec = (tw / 2) % 4 == 0 && S.step % 16 == 0 &&
+                    reinterpret_cast<uintptr_t>(S.ptr) % 16 == 0;
    hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.lds_bytes),
              "lds attribute");
    dim3 grid((S.width + kTileCols - 1) / kTileCols, (S.height + kTileRows - 1) / kTileRows);
    kern<<<grid, 256, g.lds_bytes, s>>>(static_cast<const float *>(S.ptr), S.width, S.height,
                                        S.step, tw, th, g.nub, f0, f1, tsums, integ, dst, dstep,
-                                       g.vb, g.ldsp, g.lcols);
+                                       g.vb, g.ldsp, g.lcols, vec);
 }
 
 template <int P, bool NORM>

@@ -124,12 +124,20 @@ struct View {
    int channels, type;
 };
 
+// Internal exclusive FP64 integral (window sums for the Norm ops and gradients): H+1 rows of
+// excl_pitch(W) doubles; entry (j, i) = sum over rows < j and columns < i (0 <= i <= W)
+// sits at j * pitch + i + kExclOffset, so that entries 1..W of a row start on a 32-byte
+// boundary and a row is written with full 32-byte stores.
+constexpr int kExclOffset = 3;
+__host__ __device__ inline size_t excl_pitch(int W) { return ((size_t)W + kExclOffset + 1 + 3) / 4 * 4; }
+inline size_t excl_bytes(int W, int H) { return excl_pitch(W) * ((size_t)H + 1) * sizeof(double); }
+
 // ---------------------------------------------------------------------------------------
 // Kernel launchers (defined in the .hip files).  All enqueue on `s` and return immediately.
 
 // Integral of f(src) with f = identity or square, FP64 sums.
 //   out_type F32/F64 with inclusive same-size layout (public API), or
-//   out_type = -1: exclusive padded layout (W+1) x (H+1) doubles (internal, for window sums).
+//   out_type = -1: the internal exclusive layout above (out_step ignored).
 size_t integral_aux_bytes(int width, int height);
 void launch_integral(hipStream_t s, const View &src, int channel, bool sqr, void *out,
                      int out_step, int out_type, void *aux);

@@ -62,15 +62,6 @@ __global__ __launch_bounds__(256) void template_prep(const char *tpl, int tw, in
    }
 }
 
-// window sum of s^2 for output (x, y) from the exclusive padded integral
-__device__ __forceinline__ double window_sqsum(const double *__restrict__ I, int W, int H,
-                                               int x, int y, int tw, int th)
-{
-   const int L = min(max(x - tw / 2, 0), W), R = min(max(x - tw / 2 + tw, 0), W);
-   const int T = min(max(y - th / 2, 0), H), B = min(max(y - th / 2 + th, 0), H);
-   const size_t P = (size_t)W + 1;
-   return (I[B * P + R] - I[T * P + R]) - (I[B * P + L] - I[T * P + L]);
-}
 
 // ------------------------------------------------------------------------- per-tap maths
 template <int OP> __device__ __forceinline__ float tap(float acc, float s, float t)
@@ -112,14 +103,21 @@ __device__ __forceinline__ void stage_tile_f32(float *lds, int ldsw, int nrows, 
                                                const char *src, int W, int H, int sstep,
                                                int nch, int ch, int sx0, int sy0)
 {
+   // eight elements per thread in flight
    const int n = nrows * ldsw;
-   for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      const int r = i / ldsw, c = i - r * ldsw;
-      const int sx = sx0 + c, sy = sy0 + r;
-      float v = 0.f;
-      if (c < ncols && sx >= 0 && sx < W && sy >= 0 && sy < H)
-         v = (float)ld<TYPE>(src + (size_t)sy * sstep, sx * nch + ch);
-      lds[i] = v;
+   for (int i0 = threadIdx.x; i0 < n; i0 += 256 * 8) {
+      float v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+         const int i = i0 + 256 * k;
+         const int r = i / ldsw, c = i - r * ldsw;
+         const int sx = sx0 + c, sy = sy0 + r;
+         v[k] = (i < n && c < ncols && sx >= 0 && sx < W && sy >= 0 && sy < H)
+                    ? (float)ld<TYPE>(src + (size_t)sy * sstep, sx * nch + ch) : 0.f;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+         if (i0 + 256 * k < n) lds[i0 + 256 * k] = v[k];
    }
 }
 
@@ -186,13 +184,19 @@ __device__ __forceinline__ void stage_tile_int(int32_t *lds, int ldsw, int nrows
                                                int nch, int ch, int sx0, int sy0)
 {
    const int n = nrows * ldsw;
-   for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      const int r = i / ldsw, c = i - r * ldsw;
-      const int sx = sx0 + c, sy = sy0 + r;
-      int32_t v = 0;
-      if (c < ncols && sx >= 0 && sx < W && sy >= 0 && sy < H)
-         v = (int32_t)ld<TYPE>(src + (size_t)sy * sstep, sx * nch + ch);
-      lds[i] = v;
+   for (int i0 = threadIdx.x; i0 < n; i0 += 256 * 8) {
+      int32_t v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+         const int i = i0 + 256 * k;
+         const int r = i / ldsw, c = i - r * ldsw;
+         const int sx = sx0 + c, sy = sy0 + r;
+         v[k] = (i < n && c < ncols && sx >= 0 && sx < W && sy >= 0 && sy < H)
+                    ? (int32_t)ld<TYPE>(src + (size_t)sy * sstep, sx * nch + ch) : 0;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+         if (i0 + 256 * k < n) lds[i0 + 256 * k] = v[k];
    }
 }
 
