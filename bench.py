@@ -65,6 +65,30 @@ def time_steps(ocip, fn, steps, warmup, kernel):
     return (t1 - t0) * 1e3 / steps, (kms / kn if kn else None)
 
 
+def batch_bench(ocip, steps):
+    """256 templates with sides 8..96 against one 2048x2048 F32 image, CrossCorr_Norm, one
+    ocipCrossCorr_NormBatch call per step (BF16 MFMA, one recorded graph), next to the plain
+    FP32 path on the same batch."""
+    from openclipp_amd import workloads
+    src, tpls, _ = workloads.batch_2048()
+    H, W = src.shape
+    tpix = sum(W * H * t.shape[0] * t.shape[1] for t in tpls)
+    S = ocip.Image(src).send()
+    Ts = [ocip.Image(t).send() for t in tpls]
+    Ds = [ocip.Image(shape=(H, W), dtype=np.float32, device_only=True) for _ in tpls]
+    th, dh = ocip.handles(Ts), ocip.handles(Ds)
+    out = {"workload": f"CrossCorr_Norm, {W}x{H} F32 U[0,1), {len(tpls)} templates 8..96 px",
+           "tpix_per_step": tpix}
+    for name, prec, n in (("bf16", 1, steps), ("f32", 0, 1)):
+        fn = lambda: ocip.lib.ocipCrossCorr_NormBatch(S.handle, th, dh, len(Ts), prec)
+        ms, kms = time_steps(ocip, fn, n, 1, "crosscorr_mfma_batch" if prec else "prox_tile_f32")
+        out[name] = {"value": round(tpix / (ms * 1e-3) / 1e9, 3), "ms_per_step": round(ms, 3)}
+    out["speedup_bf16_vs_f32"] = round(out["bf16"]["value"] / out["f32"]["value"], 3)
+    for i in [S] + Ts + Ds:
+        i.release()
+    return out
+
+
 def cpu_baseline(src, tpl, target_s=12.0):
     from oracle import oracle as orc
     nthreads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))))
@@ -155,6 +179,7 @@ def main():
                       "max_abs_err": float(err32.max()), "mean_abs_err": float(err32.mean())},
         "speedup_vs_plain_f32": round(value / value32, 3),
     }
+    line["batch_2048"] = batch_bench(ocip, max(2, args.steps // 4))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(src, tpl)
     for i in (S, T, D, D32):

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -32,11 +34,18 @@ bool live(Context *c)
    return false;
 }
 
+bool verbose()
+{
+   static const bool v = getenv("OCIP_VERBOSE") != nullptr;
+   return v;
+}
+
 template <class F> cl_int guarded(F &&f)
 {
    try {
       f();
    } catch (const Error &e) {
+      if (verbose()) fprintf(stderr, "ocip: %s (%d)\n", e.what, e.code);
       return e.code;
    } catch (const std::bad_alloc &) {
       return CL_OUT_OF_HOST_MEMORY;
@@ -200,7 +209,54 @@ void run_backward(int mode, ocipImage hs, ocipImage ht, ocipImage hg, ocipImage 
    if (dT) dT->in_device = true;
 }
 
+// Many templates against one source (extensions).  F32 precision runs the plain path per
+// template; the MFMA precisions run one recorded graph for the whole batch.
+void run_batch(bool ssd, ocipImage hs, const ocipImage *ht, ocipImage *hd, uint count,
+               int precision)
+{
+   Context &c = current();
+   Image &S = img(hs);
+   if (count > 0 && (!ht || !hd)) fail(CL_INVALID_VALUE, "null template or dest array");
+   throw_if(check::precision(S.desc, precision), "precision");
+   if (S.desc.Type != SImage::F32 || S.desc.Channels != 1)
+      fail(CL_INVALID_VALUE, "batches need F32 1-channel images");
+   same_context(c, S);
+   for (uint i = 0; i < count; ++i) {
+      Image &T = img(ht[i]), &D = img(hd[i]);
+      throw_if(check::proximity(S.desc, T.desc, D.desc), "proximity arguments");
+      if (T.desc.Width == 0 || T.desc.Height == 0) fail(CL_INVALID_VALUE, "empty template");
+      same_context(c, T);
+      same_context(c, D);
+   }
+   if (count == 0) return;
+   if (precision == ocipPrecisionF32) {
+      for (uint i = 0; i < count; ++i)
+         run_proximity(ssd ? kSSDNorm : kCCNorm, hs, ht[i], hd[i], ocipPrecisionF32);
+      return;
+   }
+   set_device(c);
+   send_if_needed(c, S);
+   std::vector<View> tv;
+   std::vector<float *> dv;
+   std::vector<int> ds;
+   for (uint i = 0; i < count; ++i) {
+      Image &T = img(ht[i]), &D = img(hd[i]);
+      send_if_needed(c, T);
+      tv.push_back(view(T));
+      dv.push_back(static_cast<float *>(D.dev));
+      ds.push_back((int)D.desc.Step);
+   }
+   if (!c.batch) c.batch = batch_cache_create();
+   run_batch_mfma(c.stream, c.batch, c.num_cus, precision, ssd, view(S), tv, dv, ds);
+   for (uint i = 0; i < count; ++i) img(hd[i]).in_device = true;
+}
+
 }  // namespace
+
+void ocip::report_hip_error(hipError_t e, const char *what)
+{
+   if (verbose()) fprintf(stderr, "ocip: HIP error in %s: %s\n", what, hipGetErrorString(e));
+}
 
 void DevBuf::ensure(size_t n)
 {
@@ -273,6 +329,8 @@ ocipError ocip_API ocipUninitialize(ocipContext Context)
       c->integ_aux.release();
       c->stats.release();
       c->work.release();
+      batch_cache_destroy(c->batch);
+      c->batch = nullptr;
       (void)hipStreamDestroy(c->stream);
       c->magic = 0;
       delete c;
@@ -492,6 +550,18 @@ ocipError ocip_API ocipCrossCorr_NormEx(ocipImage Source, ocipImage Template, oc
                                         int Precision)
 {
    return guarded([&] { run_proximity(kCCNorm, Source, Template, Dest, Precision); });
+}
+
+ocipError ocip_API ocipCrossCorr_NormBatch(ocipImage Source, const ocipImage *Templates,
+                                           ocipImage *Dests, uint Count, int Precision)
+{
+   return guarded([&] { run_batch(false, Source, Templates, Dests, Count, Precision); });
+}
+
+ocipError ocip_API ocipSqrDistance_NormBatch(ocipImage Source, const ocipImage *Templates,
+                                             ocipImage *Dests, uint Count, int Precision)
+{
+   return guarded([&] { run_batch(true, Source, Templates, Dests, Count, Precision); });
 }
 
 ocipError ocip_API ocipCrossCorr_NormBackward(ocipImage Source, ocipImage Template,

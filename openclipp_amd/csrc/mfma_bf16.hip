@@ -1,0 +1,3 @@
+// Kernel instances of the MFMA cross-correlation for one operand precision.
+#define OCIP_MFMA_P 1
+#include "mfma_inst.h"

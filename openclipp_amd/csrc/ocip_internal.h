@@ -13,6 +13,7 @@
 #include <stdint.h>
 #include <stddef.h>
 #include <string>
+#include <vector>
 
 #include "OpenCLIPP.h"
 
@@ -30,10 +31,15 @@ struct Error {
 
 [[noreturn]] inline void fail(cl_int code, const char *what) { throw Error{code, what}; }
 
+// Failing HIP call: the error is mapped to an OpenCL code; with OCIP_VERBOSE set in the
+// environment the HIP error text is printed to stderr.
+void report_hip_error(hipError_t e, const char *what);
 inline void hip_check(hipError_t e, const char *what)
 {
-   if (e != hipSuccess) fail(e == hipErrorOutOfMemory ? CL_MEM_OBJECT_ALLOCATION_FAILURE
-                                                      : CL_OUT_OF_RESOURCES, what);
+   if (e != hipSuccess) {
+      report_hip_error(e, what);
+      fail(e == hipErrorOutOfMemory ? CL_MEM_OBJECT_ALLOCATION_FAILURE : CL_OUT_OF_RESOURCES, what);
+   }
 }
 
 // Growable device buffer (contents are not preserved by `ensure`).
@@ -46,6 +52,7 @@ struct DevBuf {
 };
 
 struct Context;
+struct BatchCache;
 
 enum : uint32_t { kImageMagic = 0x474d494fu /*OIMG*/, kContextMagic = 0x5854434fu /*OCTX*/,
                   kProgramMagic = 0x4752504fu /*OPRG*/ };
@@ -86,11 +93,14 @@ struct Context {
    DevBuf integ;          // FP64 exclusive sum-of-squares integral, (W+1) x (H+1)
    DevBuf integ_aux;      // integral scan aggregates
    DevBuf stats;          // reduction partials for MinIndx / MaxIndx, MFMA fragments
-   DevBuf work;           // gradient and batch workspaces
+   DevBuf work;           // gradient workspace
+   BatchCache *batch = nullptr;  // recorded batch graphs (mfma.hip)
 };
 
 // Kernel timing: construct before a launch, destroy after it.  Records a HIP event pair on
 // `s` when timing is enabled (ocipEnableKernelTiming), nothing otherwise.
+// While suspended (stream capture), no timing events are recorded.
+void timing_suspend(bool suspend);
 struct KernelTimer {
    KernelTimer(hipStream_t s, const char *name);
    ~KernelTimer();
@@ -164,6 +174,15 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
                            int tw, int th, const float *tstage, int tpitch, const double *tsums,
                            const double *integ, float *dst, int dst_step, void *work);
 size_t crosscorr_mfma_work_bytes(int precision, int width, int height, int tw, int th);
+
+// Batched MFMA CrossCorr_Norm (ssd = false) or SqrDistance_Norm (ssd = true) of F32 1-channel
+// templates against one F32 1-channel source, recorded into a HIP graph on first use and
+// replayed for identical arguments.  The cache belongs to the context.
+BatchCache *batch_cache_create();
+void batch_cache_destroy(BatchCache *c);
+void run_batch_mfma(hipStream_t s, BatchCache *cache, int num_cus, int precision, bool ssd,
+                    const View &S, const std::vector<View> &T, const std::vector<float *> &dst,
+                    const std::vector<int> &dstep);
 
 // Gradients (backward.hip).  mode 0: CrossCorr_Norm, 1: SqrDistance.  dS / dT may be null.
 size_t backward_work_bytes(int W, int H, int tw, int th, int tpitch);

@@ -15,6 +15,7 @@ struct Record {
    hipEvent_t start, stop;
 };
 bool g_enabled = false;
+bool g_suspended = false;
 std::vector<Record> g_records;
 
 void clear_records()
@@ -27,9 +28,11 @@ void clear_records()
 }
 }  // namespace
 
+void timing_suspend(bool suspend) { g_suspended = suspend; }
+
 KernelTimer::KernelTimer(hipStream_t s, const char *n) : stream(s), name(n)
 {
-   if (!g_enabled) return;
+   if (!g_enabled || g_suspended) return;
    hipEvent_t e;
    if (hipEventCreate(&e) != hipSuccess) return;
    if (hipEventRecord(e, s) != hipSuccess) {
