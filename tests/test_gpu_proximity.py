@@ -233,3 +233,22 @@ def test_precision_4096_plain_bands(ocip):
     for y0, y1 in [(0, 2), (2047, 2049), (4094, 4096)]:
         want = band_oracle(lambda c: orc.prox_ref(orc.CC, c, tpl, nthreads=8), src, tpl, y0, y1)
         np.testing.assert_array_equal(cc[y0:y1], want)
+
+
+def test_golden_fixtures_through_the_abi(ocip):
+    """The committed numpy brute-force fixtures (tests/golden) through the C ABI."""
+    import glob
+    import os
+    for path in sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz"))):
+        g = np.load(path)
+        src, tpl = g["src"], g["tpl"]
+        n = tpl.size
+        for op in RAW_OPS + NORM_OPS:
+            got = run_op(ocip, op, src, tpl).astype(np.float64)
+            if np.issubdtype(src.dtype, np.integer) and op in RAW_OPS:
+                np.testing.assert_array_equal(got, g[op + "_i64"].astype(np.float32), err_msg=path)
+            elif src.dtype == np.float64:
+                np.testing.assert_allclose(got, g[op], rtol=1e-12, atol=1e-12, err_msg=path)
+            else:
+                bound = (3 * (n + 4) * 2.0 ** -24 + 2.0 ** -40) * np.abs(g[op]) + 1e-6
+                assert np.all(np.abs(got - g[op]) <= bound), (path, op)
