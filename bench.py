@@ -89,6 +89,21 @@ def batch_bench(ocip, steps):
     return out
 
 
+def copy_calibration():
+    """One device-to-device copy of 256 MiB by a torch elementwise kernel: its FETCH_SIZE and
+    WRITE_SIZE counter readings (KiB per dispatch) against the 262144 KiB moved give the
+    scale that turns those counters into HBM bytes."""
+    import torch
+    n = 64 * 1024 * 1024  # float32 elements = 256 MiB
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4}), flush=True)
+    return 0
+
+
 def cpu_baseline(src, tpl, target_s=12.0):
     from oracle import oracle as orc
     nthreads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))))
@@ -116,7 +131,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="bf16", choices=list(PREC_CODE))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 256 MiB device buffer (scale for rocprofv3 FETCH_SIZE/"
+                         "WRITE_SIZE readings)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the template-"
               "matching path has no multi-GPU mode in this build.", file=sys.stderr)
@@ -140,6 +161,10 @@ def main():
     prec = args.precision
     ms, kms = time_steps(ocip, step(PREC_CODE[prec]), args.steps, args.warmup, DOMINANT[prec])
     value = tpix / (ms * 1e-3) / 1e9
+    # the same contraction without the Norm epilogue (ocipCrossCorrEx)
+    ms_cc, kms_cc = time_steps(ocip, lambda: ocip.lib.ocipCrossCorrEx(S.handle, T.handle, D32.handle,
+                                                                      PREC_CODE[prec]),
+                               max(3, args.steps // 2), 1, DOMINANT[prec])
     # plain FP32 VALU path on the same inputs
     ms32, kms32 = time_steps(ocip, step(0), max(3, args.steps // 2), 1, DOMINANT["f32"])
     value32 = tpix / (ms32 * 1e-3) / 1e9
@@ -178,8 +203,12 @@ def main():
                       "achieved_tflops": round(flops / (kms32 * 1e-3) / 1e12, 2) if kms32 else None,
                       "max_abs_err": float(err32.max()), "mean_abs_err": float(err32.mean())},
         "speedup_vs_plain_f32": round(value / value32, 3),
+        "crosscorr_same_precision": {"value": round(tpix / (ms_cc * 1e-3) / 1e9, 3),
+                                     "ms_per_step": round(ms_cc, 4),
+                                     "kernel_ms": round(kms_cc, 4) if kms_cc else None},
     }
-    line["batch_2048"] = batch_bench(ocip, max(2, args.steps // 4))
+    if not args.no_batch:
+        line["batch_2048"] = batch_bench(ocip, max(2, args.steps // 4))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(src, tpl)
     for i in (S, T, D, D32):

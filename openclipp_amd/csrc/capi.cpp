@@ -146,15 +146,15 @@ void run_proximity(int op, ocipImage hs, ocipImage ht, ocipImage hd, int precisi
 
    for (int ch = 0; ch < nch; ++ch) {
       const double *integ = norm ? sqr_integral_for(c, S, ch) : nullptr;
-      launch_template_prep(c.stream, view(T), ch, tpitch, kind, c.tmpl.data.ptr,
-                           c.tmpl.sums.as<double>());
       if (precision != ocipPrecisionF32) {
          size_t wb = crosscorr_mfma_work_bytes(precision, S.desc.Width, S.desc.Height, tw, th);
          c.stats.ensure(wb > 0 ? wb : 1);
-         launch_crosscorr_mfma(c.stream, precision, norm, view(S), tw, th,
-                               c.tmpl.data.as<float>(), tpitch, c.tmpl.sums.as<double>(),
-                               integ, (float *)D.dev, D.desc.Step, c.stats.ptr);
+         launch_crosscorr_mfma(c.stream, precision, norm, view(S), view(T),
+                               c.tmpl.sums.as<double>(), integ, (float *)D.dev, D.desc.Step,
+                               c.stats.ptr);
       } else {
+         launch_template_prep(c.stream, view(T), ch, tpitch, kind, c.tmpl.data.ptr,
+                              c.tmpl.sums.as<double>());
          launch_proximity(c.stream, op, view(S), view(T), D.dev, D.desc.Step, c.tmpl.data.ptr,
                           tpitch, c.tmpl.sums.as<double>(), integ, ch);
       }

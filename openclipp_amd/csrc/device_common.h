@@ -60,6 +60,19 @@ template <class T> __device__ __forceinline__ T wave_sum(T v)
 
 inline void launch_check(const char *what) { hip_check(hipGetLastError(), what); }
 
+// XCD-aware tile order.  The 8 XCDs of an MI355X each have their own L2 and receive
+// workgroups round-robin in dispatch order (workgroup b runs on XCD b % 8).  Remapping
+// workgroup b to tile (b % 8) * (n / 8) + b / 8 gives every XCD one contiguous run of tiles
+// (row-major), so tiles that share halo rows and integral rows share an L2.  Bijective on
+// [0, n): the last n % 8 workgroups keep their own index.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int b, int n)
+{
+   const int q = n / kXcds;
+   if (b >= q * kXcds) return b;
+   return (b % kXcds) * q + b / kXcds;
+}
+
 // Sum over columns [xa, xb) and rows [ya, yb), clamped to the image, from the internal
 // exclusive FP64 integral (layout: excl_pitch / kExclOffset in ocip_internal.h).
 __device__ __forceinline__ double rect_sum(const double *__restrict__ I, int W, int H, int xa,
@@ -70,6 +83,15 @@ __device__ __forceinline__ double rect_sum(const double *__restrict__ I, int W, 
    const size_t P = excl_pitch(W);
    const double *r0 = I + ya * P + kExclOffset, *r1 = I + yb * P + kExclOffset;
    return (r1[xb] - r0[xb]) - (r1[xa] - r0[xa]);
+}
+
+// 1 / sqrt(p) for the Norm epilogues: p (= window sum s^2 * sum t^2) is formed in FP64; the
+// reciprocal square root is taken in FP32 (v_rsq_f32, about 1 ulp) when p is in float range,
+// in FP64 otherwise.  p = 0 gives +inf, as the reference's x / sqrt(0) does.
+__device__ __forceinline__ float inv_sqrt(double p)
+{
+   if (p >= 1e-30 && p <= 1e30) return __builtin_amdgcn_rsqf((float)p);
+   return (float)(1.0 / sqrt(p));
 }
 
 // Window sum of the Norm denominators: the template window of output (x, y) with origin at

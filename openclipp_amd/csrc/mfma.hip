@@ -31,6 +31,20 @@ __global__ void reset_counters(unsigned *c, int n)
    if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
 }
 
+// 1 / sqrt(SS TT) for every output of a CrossCorr_Norm map: SS from four corners of the FP64
+// integral (the Norm's variance term), TT = sum t^2 in FP64, the reciprocal square root in
+// FP32 (inv_sqrt).  Rows are handed out in XCD-aware order so that the integral rows an
+// output row needs (y - th/2 and y - th/2 + th) are shared in L2 by neighbouring rows.
+__global__ __launch_bounds__(256) void ncc_inv_den(const double *__restrict__ integ, int W, int H,
+                                                   int tw, int th, const double *__restrict__ tsums,
+                                                   float *__restrict__ out, int blocks_x)
+{
+   const int b = xcd_tile(blockIdx.x, gridDim.x);
+   const int y = b / blocks_x, x = (b % blocks_x) * 256 + threadIdx.x;
+   if (y >= H || x >= W) return;
+   out[(size_t)y * W + x] = inv_sqrt(window_sqsum(integ, W, H, x, y, tw, th) * tsums[0]);
+}
+
 int kc_for(int tw) { return tw <= 97 ? std::max(2, (tw + 31 + 15) / 16) : 6; }
 int nub_for(int tw) { return tw <= 97 ? 1 : (tw + mfma::kBlockW - 1) / mfma::kBlockW; }
 int planes_for(int precision) { return precision == mfma::kBF16x3 ? 2 : 1; }
@@ -152,15 +166,22 @@ std::string tune_key(const char *kind, int precision, int KC, int mode, int W, i
 }  // namespace
 
 // ======================================================================= single template
-size_t crosscorr_mfma_work_bytes(int precision, int, int, int tw, int th)
+static size_t mfma_frag_bytes(int precision, int tw, int th)
 {
-   return (size_t)planes_for(precision) * nub_for(tw) * th * kc_for(tw) * 64 * sizeof(u16x8);
+   const size_t b = (size_t)planes_for(precision) * nub_for(tw) * th * kc_for(tw) * 64 * sizeof(u16x8);
+   return (b + 255) / 256 * 256;
 }
 
-void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &S, int tw, int th,
-                           const float *tstage, int tpitch, const double *tsums,
-                           const double *integ, float *dst, int dstep, void *work)
+// fragments, then the W x H map of 1 / sqrt(SS TT) of the Norm
+size_t crosscorr_mfma_work_bytes(int precision, int W, int H, int tw, int th)
 {
+   return mfma_frag_bytes(precision, tw, th) + (size_t)W * H * sizeof(float);
+}
+
+void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &S, const View &T,
+                           double *tsums, const double *integ, float *dst, int dstep, void *work)
+{
+   const int tw = T.width, th = T.height;
    if (tw <= 0 || th <= 0) fail(CL_INVALID_VALUE, "empty template");
    const int KC = kc_for(tw), nub = nub_for(tw);
    if (geometry(precision, KC, 4, th).vb < 1)
@@ -169,15 +190,21 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
    u16x8 *f0 = static_cast<u16x8 *>(work);
    u16x8 *f1 = precision == mfma::kBF16x3 ? f0 + nfrag * 64 : nullptr;
    {
-      KernelTimer kt(s, "toeplitz_prep");
-      if (precision == mfma::kBF16)
-         mfma::launch_toeplitz_prep<mfma::kBF16>(s, tstage, tpitch, tw, th, KC, nub, f0, f1);
-      else if (precision == mfma::kF16)
-         mfma::launch_toeplitz_prep<mfma::kF16>(s, tstage, tpitch, tw, th, KC, nub, f0, f1);
-      else
-         mfma::launch_toeplitz_prep<mfma::kBF16x3>(s, tstage, tpitch, tw, th, KC, nub, f0, f1);
+      KernelTimer kt(s, "template_prep");
+      const BatchDesc d{static_cast<const char *>(T.ptr), tw, th, T.step, KC, nub, f0, f1, tsums};
+      if (precision == mfma::kBF16) mfma::launch_single_prep<mfma::kBF16>(s, d);
+      else if (precision == mfma::kF16) mfma::launch_single_prep<mfma::kF16>(s, d);
+      else mfma::launch_single_prep<mfma::kBF16x3>(s, d);
    }
-   const int mode = norm ? mfma::kModeNCC : mfma::kModeCC;
+   const int mode = norm ? mfma::kModeNCCMap : mfma::kModeCC;
+   float *invden = nullptr;
+   if (norm) {
+      invden = reinterpret_cast<float *>(static_cast<char *>(work) + mfma_frag_bytes(precision, tw, th));
+      KernelTimer kt(s, "ncc_inv_den");
+      const int bx = (S.width + 255) / 256;
+      ncc_inv_den<<<bx * S.height, 256, 0, s>>>(integ, S.width, S.height, tw, th, tsums, invden, bx);
+      launch_check("ncc_inv_den");
+   }
    // staged columns start at x0 - tw/2 + 64 ub + 8 g with x0 a multiple of 64: 16-byte
    // aligned when tw/2 is a multiple of 4 and the rows are 16-byte aligned
    const int vec = ((tw / 2) % 4 == 0 && rows_aligned(S)) ? 1 : 0;
@@ -185,7 +212,7 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
       const Geometry g = geometry(precision, KC, NT, th);
       SingleLaunch L;
       L.a = mfma::TileArgs{static_cast<const float *>(S.ptr), S.width, S.height, S.step, tw, th,
-                           nub, f0, f1, tsums, integ, dst, dstep, g.vb, vec};
+                           nub, f0, f1, tsums, integ, invden, dst, dstep, g.vb, vec};
       L.KC = KC;
       L.NT = NT;
       L.mode = mode;

@@ -24,7 +24,8 @@ template <int KC, int NT, int MODE> void single(hipStream_t s, const SingleLaunc
 {
    auto k = crosscorr_mfma<P, KC, NT, MODE>;
    allow_lds(k);
-   k<<<L.grid, 256, L.lds, s>>>(L.a, L.ldsp);
+   const int tiles = L.grid.x * L.grid.y;
+   k<<<tiles, 256, L.lds, s>>>(L.a, L.ldsp, (int)L.grid.x, tiles);
 }
 
 template <int KC, int NT, int MODE> void batch(hipStream_t s, const BatchLaunch &L)
@@ -47,7 +48,7 @@ template <int KC, int MODE> void batch_nt(hipStream_t s, const BatchLaunch &L)
 
 template <int KC> void single_mode(hipStream_t s, const SingleLaunch &L)
 {
-   if (L.mode == kModeCC) single_nt<KC, kModeCC>(s, L); else single_nt<KC, kModeNCC>(s, L);
+   if (L.mode == kModeCC) single_nt<KC, kModeCC>(s, L); else single_nt<KC, kModeNCCMap>(s, L);
 }
 
 template <int KC> void batch_mode(hipStream_t s, const BatchLaunch &L)
@@ -87,12 +88,11 @@ template <> void launch_batch<P>(hipStream_t s, const BatchLaunch &L)
    launch_check("crosscorr_mfma_batch");
 }
 
-template <> void launch_toeplitz_prep<P>(hipStream_t s, const float *tstage, int tpitch, int tw,
-                                         int th, int KC, int nub, u16x8 *f0, u16x8 *f1)
+template <> void launch_single_prep<P>(hipStream_t s, const BatchDesc &d)
 {
-   const int nfrag = nub * th * KC;
-   toeplitz_prep<P><<<(nfrag + 3) / 4, 256, 0, s>>>(tstage, tpitch, tw, th, KC, nub, f0, f1);
-   launch_check("toeplitz_prep");
+   const int nfrag = d.nub * d.th * d.KC;
+   single_prep<P><<<(nfrag + 3) / 4, 256, 0, s>>>(d);
+   launch_check("single_prep");
 }
 
 template <> void launch_batch_prep<P>(hipStream_t s, const BatchDesc *descs, int n)

@@ -42,7 +42,8 @@ constexpr int kBlockW = 64;             // template column block for wide templa
 constexpr int kLdsMax = 160 * 1024 - 256;
 
 enum { kBF16 = 1, kF16 = 2, kBF16x3 = 3 };
-enum { kModeCC = 0, kModeNCC = 1, kModeSSDN = 2 };
+// kModeNCCMap: CrossCorr_Norm with 1 / sqrt(SS TT) precomputed per output (ncc_inv_den)
+enum { kModeCC = 0, kModeNCC = 1, kModeSSDN = 2, kModeNCCMap = 3 };
 
 // Arguments of one output tile computation.
 struct TileArgs {
@@ -52,6 +53,7 @@ struct TileArgs {
    const u16x8 *frag0, *frag1;   // B fragments (plane 1: lo parts for split BF16)
    const double *tsums;          // [0] = sum t^2
    const double *integ;          // internal exclusive FP64 integral of s^2 (Norm modes)
+   const float *invden;          // 1 / sqrt(SS TT) per output, pitch W (kModeNCCMap)
    float *dst;
    int dstep;
    int vb;                       // template rows per LDS band
@@ -131,19 +133,6 @@ __device__ __forceinline__ void build_fragment(const char *tptr, int tstep, int 
    if (P == kBF16x3) frag1[(size_t)f * 64 + lane] = b;
 }
 
-// Single template: fragments from the staged float template (pitch in floats).
-template <int P>
-__global__ __launch_bounds__(256) void toeplitz_prep(const float *__restrict__ tstage,
-                                                     int tpitch, int tw, int th, int KC,
-                                                     int nub, u16x8 *frag0, u16x8 *frag1)
-{
-   const int nfrag = nub * th * KC;
-   const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
-   if (f >= nfrag) return;
-   build_fragment<P>(reinterpret_cast<const char *>(tstage), tpitch * 4, tw, th, KC, nub, f,
-                     threadIdx.x & 63, frag0, frag1);
-}
-
 // Batch: one workgroup per template builds its fragments and its FP64 sum of squares.
 template <int P>
 __global__ __launch_bounds__(256) void batch_prep(const BatchDesc *__restrict__ descs)
@@ -156,6 +145,31 @@ __global__ __launch_bounds__(256) void batch_prep(const BatchDesc *__restrict__ 
                         d.frag1);
    // same summation order as the single-template staging (rows padded to 16 columns,
    // 256 threads striding), so batch and single calls give identical sums
+   const int pitch = (d.tw + 15) / 16 * 16;
+   double s2 = 0;
+   for (int k = threadIdx.x; k < pitch * d.th; k += 256) {
+      const int v = k / pitch, u = k - v * pitch;
+      const double t = u < d.tw ? reinterpret_cast<const float *>(d.tptr + (size_t)v * d.tstep)[u] : 0.0;
+      s2 += t * t;
+   }
+   s2 = wave_sum(s2);
+   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s2;
+   __syncthreads();
+   if (threadIdx.x == 0) d.sums[0] = red[0] + red[1] + red[2] + red[3];
+}
+
+// Single template straight from its image: fragments in every block, the FP64 sum of
+// squares in block 0 (same order as batch_prep).
+template <int P>
+__global__ __launch_bounds__(256) void single_prep(BatchDesc d)
+{
+   __shared__ double red[4];
+   const int nfrag = d.nub * d.th * d.KC;
+   const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
+   if (f < nfrag)
+      build_fragment<P>(d.tptr, d.tstep, d.tw, d.th, d.KC, d.nub, f, threadIdx.x & 63, d.frag0,
+                        d.frag1);
+   if (blockIdx.x != 0) return;
    const int pitch = (d.tw + 15) / 16 * 16;
    double s2 = 0;
    for (int k = threadIdx.x; k < pitch * d.th; k += 256) {
@@ -307,36 +321,57 @@ __device__ __forceinline__ void mfma_tile(const TileArgs &a, int bx, int by, uns
    }
 
    // C fragment: lane holds column n = lane % 32 of each n-tile, rows
-   // m = 8 * (i / 4) + 4 * (lane / 32) + i % 4
-   const double tt = MODE != kModeCC ? a.tsums[0] : 0.0;
+   // m = 8 * (i / 4) + 4 * (lane / 32) + i % 4.  Per n-tile, the 64 integral corners of the
+   // lane's 16 outputs are loaded before any of them is used or stored.
    const int n = lane & 31, mb = 4 * (lane >> 5);
+   const int ybase = y0 + wave * 32 + mb;
+   double tt = 0.0;
+   if (MODE == kModeNCC || MODE == kModeSSDN) tt = a.tsums[0];
 #pragma unroll
    for (int j = 0; j < NT; ++j) {
       const int x = x0 + 32 * j + n;
+      float val[16];
+      if (MODE == kModeNCCMap) {
+         const int xc = x < W ? x : W - 1;
+#pragma unroll
+         for (int i = 0; i < 16; ++i) {
+            const int y = ybase + 8 * (i / 4) + (i % 4);
+            val[i] = acc[j][i] * a.invden[(size_t)(y < H ? y : H - 1) * W + xc];
+         }
+      } else if (MODE != kModeCC) {
+         double ss[16];
+#pragma unroll
+         for (int i = 0; i < 16; ++i)
+            ss[i] = window_sqsum(a.integ, W, H, x, ybase + 8 * (i / 4) + (i % 4), a.tw, a.th);
+#pragma unroll
+         for (int i = 0; i < 16; ++i) {
+            const double num = MODE == kModeNCC ? (double)acc[j][i] : ss[i] - 2.0 * (double)acc[j][i] + tt;
+            val[i] = (float)num * inv_sqrt(ss[i] * tt);
+         }
+      } else {
+#pragma unroll
+         for (int i = 0; i < 16; ++i) val[i] = acc[j][i];
+      }
       if (x >= W) continue;
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-         const int y = y0 + wave * 32 + 8 * (i / 4) + mb + (i % 4);
-         if (y >= H) continue;
-         float val = acc[j][i];
-         if (MODE != kModeCC) {
-            const double ss = window_sqsum(a.integ, W, H, x, y, a.tw, a.th);
-            const double num = MODE == kModeNCC ? (double)val : ss - 2.0 * (double)val + tt;
-            val = (float)(num / sqrt(ss * tt));
-         }
-         reinterpret_cast<float *>(reinterpret_cast<char *>(a.dst) + (size_t)y * a.dstep)[x] = val;
+         const int y = ybase + 8 * (i / 4) + (i % 4);
+         if (y < H)
+            reinterpret_cast<float *>(reinterpret_cast<char *>(a.dst) + (size_t)y * a.dstep)[x] = val[i];
       }
    }
 }
 
 // LDS use limits these kernels to two workgroups per CU (two waves per SIMD), so the compiler
 // may spend registers on keeping two template rows of B fragments in flight.
+// One-dimensional grid of tiles_x * tiles_y workgroups in XCD-aware order.
 template <int P, int KC, int NT, int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
-void crosscorr_mfma(TileArgs a, int ldsp)
+void crosscorr_mfma(TileArgs a, int ldsp, int tiles_x, int tiles)
 {
    extern __shared__ u16x8 lds_v[];
-   mfma_tile<P, KC, NT, MODE>(a, blockIdx.x, blockIdx.y, reinterpret_cast<unsigned short *>(lds_v),
+   const int t = xcd_tile(blockIdx.x, tiles);
+   mfma_tile<P, KC, NT, MODE>(a, t % tiles_x, t / tiles_x, reinterpret_cast<unsigned short *>(lds_v),
                               ldsp);
 }
 
@@ -363,7 +398,7 @@ void crosscorr_mfma_batch(const float *src, int W, int H, int sstep, int vec,
       const int tile = it - (it / tiles) * tiles;
       // 16-byte staging loads: aligned rows (vec) and tw/2 a multiple of 4
       TileArgs a{src, W, H, sstep, b.tw, b.th, b.nub, b.frag0, b.frag1, b.tsums, integ,
-                 b.dst, b.dstep, b.vb, (vec && (b.tw / 2) % 4 == 0) ? 1 : 0};
+                 nullptr, b.dst, b.dstep, b.vb, (vec && (b.tw / 2) % 4 == 0) ? 1 : 0};
       mfma_tile<P, KC, NT, MODE>(a, tile % tiles_x, tile / tiles_x,
                                  reinterpret_cast<unsigned short *>(lds_v), ldsp);
    }
@@ -395,8 +430,7 @@ struct BatchLaunch {
 
 template <int P> void launch_single(hipStream_t s, const SingleLaunch &L);
 template <int P> void launch_batch(hipStream_t s, const BatchLaunch &L);
-template <int P> void launch_toeplitz_prep(hipStream_t s, const float *tstage, int tpitch, int tw,
-                                           int th, int KC, int nub, u16x8 *f0, u16x8 *f1);
+template <int P> void launch_single_prep(hipStream_t s, const BatchDesc &d);
 template <int P> void launch_batch_prep(hipStream_t s, const BatchDesc *descs, int n);
 
 }  // namespace mfma

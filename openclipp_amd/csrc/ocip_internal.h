@@ -170,9 +170,10 @@ void launch_crosscorr_f32_anchored(hipStream_t s, const View &M, const float *ts
                                    int tw, int th, int ax, int ay, float *dst, int dstep);
 
 // MFMA cross-correlation (F32 1-channel source and template), precision 1..3.
+// The template's fragments and sum of squares (tsums[0]) are built from T in the call.
 void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &src,
-                           int tw, int th, const float *tstage, int tpitch, const double *tsums,
-                           const double *integ, float *dst, int dst_step, void *work);
+                           const View &T, double *tsums, const double *integ, float *dst,
+                           int dst_step, void *work);
 size_t crosscorr_mfma_work_bytes(int precision, int width, int height, int tw, int th);
 
 // Batched MFMA CrossCorr_Norm (ssd = false) or SqrDistance_Norm (ssd = true) of F32 1-channel

@@ -134,7 +134,10 @@ __global__ __launch_bounds__(256) void prox_tile_f32(const char *src, int W, int
    extern __shared__ float lds[];
    constexpr int U = 8;
    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-   const int x0 = blockIdx.x * (kWaves * R), y0 = blockIdx.y * kTileRows;
+   // one-dimensional grid in XCD-aware tile order
+   const int tiles_x = (W + kWaves * R - 1) / (kWaves * R);
+   const int tile = xcd_tile(blockIdx.x, gridDim.x);
+   const int x0 = (tile % tiles_x) * (kWaves * R), y0 = (tile / tiles_x) * kTileRows;
    const int ncols = kWaves * R + tw - 1;
    const int nfull = tw / U, rem = tw - nfull * U;
    const int colbase = wave * R;
@@ -164,16 +167,19 @@ __global__ __launch_bounds__(256) void prox_tile_f32(const char *src, int W, int
    const int oy = y0 + lane;
    if (oy >= H) return;
    float *drow = reinterpret_cast<float *>(dst + (size_t)oy * dstep);
-   const double tt = (OP == kSSDNorm || OP == kCCNorm) ? tsums[0] : 0.0;
+   if (OP == kSSDNorm || OP == kCCNorm) {
+      // all integral corners of the R outputs first, then the divisions
+      const double tt = tsums[0];
+      double ss[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) ss[r] = window_sqsum(integ, W, H, x0 + colbase + r, oy, tw, th);
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] = (float)((double)acc[r] / sqrt(ss[r] * tt));
+   }
 #pragma unroll
    for (int r = 0; r < R; ++r) {
       const int ox = x0 + colbase + r;
-      if (ox < W) {
-         float val = acc[r];
-         if (OP == kSSDNorm || OP == kCCNorm)
-            val = (float)((double)acc[r] / sqrt(window_sqsum(integ, W, H, ox, oy, tw, th) * tt));
-         drow[ox * nch + ch] = val;
-      }
+      if (ox < W) drow[ox * nch + ch] = acc[r];
    }
 }
 
@@ -254,18 +260,23 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
    const int oy = y0 + lane;
    if (oy >= H) return;
    float *drow = reinterpret_cast<float *>(dst + (size_t)oy * dstep);
-   const double tt = (OP == kSSDNorm || OP == kCCNorm) ? tsums[0] : 0.0;
+   float val[R];
+   if (OP == kSSDNorm || OP == kCCNorm) {
+      // exact sums over an FP64 division: all integral corners first
+      const double tt = tsums[0];
+      double ss[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) ss[r] = window_sqsum(integ, W, H, x0 + colbase + r, oy, tw, th);
+#pragma unroll
+      for (int r = 0; r < R; ++r) val[r] = (float)((double)acc[r] / sqrt(ss[r] * tt));
+   } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) val[r] = (float)acc[r];
+   }
 #pragma unroll
    for (int r = 0; r < R; ++r) {
       const int ox = x0 + colbase + r;
-      if (ox < W) {
-         float val;
-         if (OP == kSSDNorm || OP == kCCNorm)
-            val = (float)((double)acc[r] / sqrt(window_sqsum(integ, W, H, ox, oy, tw, th) * tt));
-         else
-            val = (float)acc[r];
-         drow[ox * nch + ch] = val;
-      }
+      if (ox < W) drow[ox * nch + ch] = val[r];
    }
 }
 
@@ -378,7 +389,7 @@ static void launch_op(hipStream_t s, const View &S, const View &T, void *dst, in
    dim3 grid((W + kWaves * R - 1) / (kWaves * R), (H + kTileRows - 1) / kTileRows);
 
 #define OCIP_TILE_F32(TT_, R_)                                                                 \
-   prox_tile_f32<OP, TT_, R_><<<grid, 256, lds, s>>>(                                         \
+   prox_tile_f32<OP, TT_, R_><<<grid.x * grid.y, 256, lds, s>>>(                                         \
        src, W, H, S.step, S.channels, ch, static_cast<const float *>(tstage), tpitch, tw, th, \
        tsums, integ, static_cast<char *>(dst), dstep, vb, ldsw, tw / 2, th / 2)
 #define OCIP_TILE_INT(TT_, R_, ACC_)                                                             \
@@ -428,7 +439,7 @@ void launch_crosscorr_f32_anchored(hipStream_t s, const View &M, const float *ts
    const size_t lds = (size_t)(kTileRows + vb - 1) * ldsw * 4;
    dim3 grid((W + kWaves * R - 1) / (kWaves * R), (H + kTileRows - 1) / kTileRows);
    KernelTimer kt(s, "prox_tile_f32");
-   prox_tile_f32<kCC, SImage::F32, R><<<grid, 256, lds, s>>>(
+   prox_tile_f32<kCC, SImage::F32, R><<<grid.x * grid.y, 256, lds, s>>>(
        static_cast<const char *>(M.ptr), W, H, M.step, 1, 0, tstage, tpitch, tw, th, nullptr,
        nullptr, reinterpret_cast<char *>(dst), dstep, vb, ldsw, ax, ay);
    launch_check("prox_tile_f32");
