@@ -13,7 +13,10 @@ reported.
 
 Roofline: the dominant kernel's algorithmic FLOPs (2 per template pixel per output pixel)
 divided by its average duration, timed with HIP events on the library's own stream
-(ocipEnableKernelTiming) over the timed region.
+(ocipEnableKernelTiming) over the timed region.  Its `traffic` is the HBM bytes per launch
+of that kernel from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) on this command,
+calibrated on a device copy of known size (--copy-calibration), kept in
+profiles/crosscorr_norm_4096_t64_traffic.md.
 
 CPU baseline: the reference-order restatement of CrossCorr_Norm (oracle/oracle.c, OpenMP over
 rows) on a bounded band of rows of the same image, on this host's cores.
@@ -21,6 +24,7 @@ rows) on a bounded band of rows of the same image, on this host's cores.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -87,6 +91,21 @@ def batch_bench(ocip, steps):
     for i in [S] + Ts + Ds:
         i.release()
     return out
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "crosscorr_norm_4096_t64_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the headline kernel (CrossCorr_Norm, BF16, this workload) from
+    the committed summary of two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE), each scaled
+    by what the counter reads for a device copy of known size; None if absent."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"HBM bytes per launch, crosscorr_mfma[^:]*: (\d+)", text)
+    return int(m.group(1)) if m else None
 
 
 def copy_calibration():
@@ -196,7 +215,8 @@ def main():
                      "achieved": round(achieved, 2) if achieved else None, "peak": peak,
                      "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4) if achieved else None,
-                     "kernel_ms": round(kms, 4) if kms else None, "traffic": None},
+                     "kernel_ms": round(kms, 4) if kms else None,
+                     "traffic": measured_traffic() if prec == "bf16" else None},
         "max_abs_err": float(err.max()), "mean_abs_err": float(err.mean()),
         "plain_f32": {"value": round(value32, 3), "ms_per_step": round(ms32, 4),
                       "kernel_ms": round(kms32, 4) if kms32 else None,

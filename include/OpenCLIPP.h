@@ -126,7 +126,7 @@ ocipError ocip_API ocipAbsDistance(ocipImage Source, ocipImage Template, ocipIma
 ocipError ocip_API ocipCrossCorr(ocipImage Source, ocipImage Template, ocipImage Dest);        /* :666 */
 ocipError ocip_API ocipCrossCorr_Norm(ocipImage Source, ocipImage Template, ocipImage Dest);   /* :669 */
 
-/* Statistics: match localisation (reference include/OpenCLIPP.h:560-561) -------------------
+/* Statistics: match localisation and template energy (reference include/OpenCLIPP.h:546-561) 
  * Minimum / maximum over all pixels of a 1-channel image and the first (row-major) pixel
  * holding it.  Program comes from ocipPrepareStatistics (may be NULL). */
 ocipError ocip_API ocipPrepareStatistics(ocipProgram * ProgramPtr, ocipImage Image); /* ref :546 */
@@ -134,6 +134,10 @@ ocipError ocip_API ocipMinIndx(ocipProgram Program, ocipImage Source, double * R
                                int * IndexX, int * IndexY);             /* ref OpenCLIPP.h:560 */
 ocipError ocip_API ocipMaxIndx(ocipProgram Program, ocipImage Source, double * Result,
                                int * IndexX, int * IndexY);             /* ref OpenCLIPP.h:561 */
+/* Sum and sum of squares of every channel: Result must hold one double per channel
+ * (reference Statistics::Sum / SumSqr over all channels, Statistics.cpp:350-370). */
+ocipError ocip_API ocipSum(ocipProgram Program, ocipImage Source, double * Result);    /* ref :552 */
+ocipError ocip_API ocipSumSqr(ocipProgram Program, ocipImage Source, double * Result); /* ref :553 */
 
 /* ===================== extensions (no counterpart in the reference) ===================== */
 

@@ -145,8 +145,10 @@ void run_proximity(int op, ocipImage hs, ocipImage ht, ocipImage hd, int precisi
    c.tmpl.sums.ensure(2 * sizeof(double));
 
    for (int ch = 0; ch < nch; ++ch) {
-      const double *integ = norm ? sqr_integral_for(c, S, ch) : nullptr;
-      if (precision != ocipPrecisionF32) {
+      const bool mfma = precision != ocipPrecisionF32;
+      const double *integ =
+          norm && (!mfma || crosscorr_mfma_needs_integral(tw)) ? sqr_integral_for(c, S, ch) : nullptr;
+      if (mfma) {
          size_t wb = crosscorr_mfma_work_bytes(precision, S.desc.Width, S.desc.Height, tw, th);
          c.stats.ensure(wb > 0 ? wb : 1);
          launch_crosscorr_mfma(c.stream, precision, norm, view(S), view(T),
@@ -613,6 +615,33 @@ ocipError ocip_API ocipMaxIndx(ocipProgram Program, ocipImage Source, double *Re
                                int *IndexX, int *IndexY)
 {
    return run_minmax(Program, Source, Result, IndexX, IndexY, true);
+}
+
+// Statistics::Sum / SumSqr (Statistics.cpp:350-370): every channel, no channel check.
+static ocipError run_sum(ocipProgram Program, ocipImage Source, double *Result, bool sqr)
+{
+   if (!g_current) return CL_INVALID_CONTEXT;
+   return guarded([&] {
+      ocip::Image &S = img(Source);
+      if (!Result) fail(CL_INVALID_VALUE, "null result pointer");
+      Context &c = Program ? *prog(Program, kProgStatistics).ctx : current();
+      same_context(c, S);
+      set_device(c);
+      send_if_needed(c, S);
+      DevBuf &buf = Program ? reinterpret_cast<ocip::Program *>(Program)->scratch : c.stats;
+      buf.ensure(sum_partials_bytes(S.desc.Width, S.desc.Height));
+      launch_sum(c.stream, sqr, view(S), buf.ptr, Result);
+   });
+}
+
+ocipError ocip_API ocipSum(ocipProgram Program, ocipImage Source, double *Result)
+{
+   return run_sum(Program, Source, Result, false);
+}
+
+ocipError ocip_API ocipSumSqr(ocipProgram Program, ocipImage Source, double *Result)
+{
+   return run_sum(Program, Source, Result, true);
 }
 
 }  // extern "C"

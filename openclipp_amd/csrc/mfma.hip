@@ -31,18 +31,170 @@ __global__ void reset_counters(unsigned *c, int n)
    if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
 }
 
-// 1 / sqrt(SS TT) for every output of a CrossCorr_Norm map: SS from four corners of the FP64
-// integral (the Norm's variance term), TT = sum t^2 in FP64, the reciprocal square root in
-// FP32 (inv_sqrt).  Rows are handed out in XCD-aware order so that the integral rows an
-// output row needs (y - th/2 and y - th/2 + th) are shared in L2 by neighbouring rows.
-__global__ __launch_bounds__(256) void ncc_inv_den(const double *__restrict__ integ, int W, int H,
-                                                   int tw, int th, const double *__restrict__ tsums,
-                                                   float *__restrict__ out, int blocks_x)
+// 1 / sqrt(SS TT) for every output of a CrossCorr_Norm map, straight from the source image:
+// SS(x, y) = sum of s^2 over the template window of (x, y) (origin (-tw/2, -th/2), zero
+// outside the image), TT = sum t^2 in FP64, the reciprocal square root in FP32 (inv_sqrt).
+// One wavefront owns kDenCols output columns and a band of kDenBand output rows:
+//  * vertical pass, lane = column of the window segment [x0 - tw/2, x0 - tw/2 + kDenCols - 1
+//    + tw): two FP64 running sums of s^2 from the band's first row, `lead` taking each row as
+//    it enters the window and `lag` as it leaves, so that the column sum V = lead - lag is the
+//    difference of two prefixes of one sequence (an all-zero column gives exactly 0).  V goes
+//    to LDS as FP32 (relative error 2^-24), kDenRows output rows at a time;
+//  * horizontal pass, lane = (output row, half of the columns): the same pair of running sums
+//    along the row of V gives SS; the results go back through LDS and leave as coalesced rows.
+// The rows are a stream of steps: step j takes row y0 - th/2 + j into `lead` (j >= 0), row
+// y0 - th/2 + j - th into `lag` (j >= th) and completes output row y0 + j - (th - 1).  Steps
+// run in batches of U, the loads of batch b + 1 in flight while batch b is summed; the
+// stream starts at jstart <= 0 (steps before 0 load nothing) so that output rows fill the
+// LDS block at batch boundaries.  No integral is formed: the source is read from HBM about
+// once (halo columns and the lag rows come from L2) and the map is written once.
+constexpr int kDenRows = 32;     // output rows per LDS block
+constexpr int kDenBand = 128;    // output rows per wavefront
+constexpr int kDenCols = 64;     // output columns per wavefront (two halves of 32)
+constexpr int kDenMaxChunks = 9; // 64-column chunks of the segment: tw <= 64 * 9 - 63
+
+__device__ float kZeroPad = 0.f;   // never written: the zero read for positions outside the image
+
+int den_chunks(int tw) { return (kDenCols - 1 + tw + 63) / 64; }
+size_t den_lds_bytes(int tw) { return (size_t)kDenRows * (den_chunks(tw) * 64 + 1) * sizeof(float); }
+
+template <int NCH>
+__global__ __launch_bounds__(64) void ncc_den_src(const char *__restrict__ src, int sstep, int W,
+                                                  int H, int tw, int th,
+                                                  const double *__restrict__ tsums,
+                                                  float *__restrict__ out, int bands)
 {
+   extern __shared__ float vbuf[];  // kDenRows x P floats; P odd: conflict-free row walks
+   constexpr int P = 64 * NCH + 1;
+   constexpr int U = NCH <= 2 ? 16 : (NCH <= 3 ? 8 : 4);   // steps per batch (divides kDenRows)
+   constexpr int HALF = kDenCols / 2;           // outputs per lane, horizontal pass
+   constexpr int BPB = kDenRows / U;            // output batches per LDS block
+   // consecutive blocks (one XCD's run) are side by side along a band: neighbours share the
+   // halo columns of the segment in L2
    const int b = xcd_tile(blockIdx.x, gridDim.x);
-   const int y = b / blocks_x, x = (b % blocks_x) * 256 + threadIdx.x;
-   if (y >= H || x >= W) return;
-   out[(size_t)y * W + x] = inv_sqrt(window_sqsum(integ, W, H, x, y, tw, th) * tsums[0]);
+   const int cblks = gridDim.x / bands;
+   const int band = b / cblks, cblk = b % cblks;
+   const int lane = threadIdx.x;
+   const int x0 = cblk * kDenCols, cx = tw / 2, cy = th / 2;
+   const int y0 = band * kDenBand, n = min(kDenBand, H - y0);
+   const double tt = tsums[0];
+   int col[NCH];
+   bool cok[NCH];
+#pragma unroll
+   for (int k = 0; k < NCH; ++k) {
+      const int c = x0 - cx + 64 * k + lane;
+      cok[k] = c >= 0 && c < W && 64 * k + lane < kDenCols - 1 + tw;
+      col[k] = min(max(c, 0), W - 1);
+   }
+   const int pre = th - 1;                       // steps before the first output row
+   const int jstart = -((U - pre % U) % U);      // (pre - jstart) % U == 0
+   const int b0 = (pre - jstart) / U;            // first batch that completes output rows
+   const int nb = b0 + (n + U - 1) / U;
+
+   // A position outside the image (or a step that takes no row) reads the zero of kZeroPad:
+   // every load is unconditional (an address select, no branch around it), so a whole batch
+   // is in flight at once.
+   auto load_batch = [&](int bb, float (&L)[U][NCH], float (&G)[U][NCH]) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+         const int j = jstart + bb * U + u;
+         const bool live = j - pre < n;
+         const int lr = y0 - cy + j, gr = lr - th;
+         const bool lok = live && j >= 0 && lr >= 0 && lr < H;
+         const bool gok = live && j >= th && gr >= 0 && gr < H;
+         const float *lrow = reinterpret_cast<const float *>(src + (size_t)min(max(lr, 0), H - 1) * sstep);
+         const float *grow = reinterpret_cast<const float *>(src + (size_t)min(max(gr, 0), H - 1) * sstep);
+#pragma unroll
+         for (int k = 0; k < NCH; ++k) {
+            L[u][k] = *((lok && cok[k]) ? lrow + col[k] : &kZeroPad);
+            G[u][k] = *((gok && cok[k]) ? grow + col[k] : &kZeroPad);
+         }
+      }
+   };
+
+   double lead[NCH], lag[NCH];
+#pragma unroll
+   for (int k = 0; k < NCH; ++k) lead[k] = lag[k] = 0.0;
+
+   // horizontal pass and stores of the nrows rows of the LDS block (output rows ybase..)
+   auto flush = [&](int ybase, int nrows) {
+      __syncthreads();
+      const int r = lane & 31, xs = (lane >> 5) * HALF;
+      float res[HALF];
+      if (r < nrows) {
+         const float *row = vbuf + r * P + xs;
+         double hl = 0.0, hg = 0.0;
+         for (int i = 0; i < tw - 1; ++i) hl += (double)row[i];
+#pragma unroll
+         for (int x = 0; x < HALF; ++x) {
+            hl += (double)row[x + tw - 1];
+            const double ss = hl - hg;
+            hg += (double)row[x];
+            res[x] = inv_sqrt(ss * tt);
+         }
+      }
+      __syncthreads();
+      if (r < nrows) {
+#pragma unroll
+         for (int x = 0; x < HALF; ++x) vbuf[r * P + xs + x] = res[x];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int h = 0; h < kDenCols / 64; ++h) {
+         const int x = x0 + 64 * h + lane;
+         if (x < W)
+            for (int rr = 0; rr < nrows; ++rr)
+               out[(size_t)(ybase + rr) * W + x] = vbuf[rr * P + 64 * h + lane];
+      }
+      __syncthreads();
+   };
+
+   auto process = [&](int bb, const float (&L)[U][NCH], const float (&G)[U][NCH]) {
+      const int e0 = (bb - b0) * U;             // output row (band-relative) of step u = 0
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+#pragma unroll
+         for (int k = 0; k < NCH; ++k) {
+            lead[k] = fma((double)L[u][k], (double)L[u][k], lead[k]);
+            lag[k] = fma((double)G[u][k], (double)G[u][k], lag[k]);
+         }
+         if (bb >= b0 && e0 + u < n) {
+#pragma unroll
+            for (int k = 0; k < NCH; ++k)
+               vbuf[((e0 + u) % kDenRows) * P + 64 * k + lane] = (float)(lead[k] - lag[k]);
+         }
+      }
+      if (bb >= b0) {
+         const int done = min(e0 + U, n);        // output rows completed so far
+         if ((bb - b0 + 1) % BPB == 0 || done == n) {
+            const int first = (done - 1) / kDenRows * kDenRows;
+            flush(y0 + first, done - first);
+         }
+      }
+   };
+
+   float La[U][NCH], Ga[U][NCH], Lb[U][NCH], Gb[U][NCH];
+   load_batch(0, La, Ga);
+   for (int bb = 0; bb < nb; bb += 2) {
+      if (bb + 1 < nb) load_batch(bb + 1, Lb, Gb);
+      process(bb, La, Ga);
+      if (bb + 1 >= nb) break;
+      if (bb + 2 < nb) load_batch(bb + 2, La, Ga);
+      process(bb + 1, Lb, Gb);
+   }
+}
+
+// Wider templates (tw > 64 * kDenMaxChunks - 63): one output per thread, four corners of the
+// FP64 integral each.
+__global__ __launch_bounds__(256) void ncc_inv_den_direct(const double *__restrict__ integ, int W,
+                                                          int H, int tw, int th,
+                                                          const double *__restrict__ tsums,
+                                                          float *__restrict__ out)
+{
+   const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+   if (k >= (long long)W * H) return;
+   const int y = (int)(k / W), x = (int)(k - (long long)y * W);
+   out[k] = inv_sqrt(window_sqsum(integ, W, H, x, y, tw, th) * tsums[0]);
 }
 
 int kc_for(int tw) { return tw <= 97 ? std::max(2, (tw + 31 + 15) / 16) : 6; }
@@ -173,6 +325,8 @@ static size_t mfma_frag_bytes(int precision, int tw, int th)
 }
 
 // fragments, then the W x H map of 1 / sqrt(SS TT) of the Norm
+bool crosscorr_mfma_needs_integral(int tw) { return den_chunks(tw) > kDenMaxChunks; }
+
 size_t crosscorr_mfma_work_bytes(int precision, int W, int H, int tw, int th)
 {
    return mfma_frag_bytes(precision, tw, th) + (size_t)W * H * sizeof(float);
@@ -201,8 +355,36 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
    if (norm) {
       invden = reinterpret_cast<float *>(static_cast<char *>(work) + mfma_frag_bytes(precision, tw, th));
       KernelTimer kt(s, "ncc_inv_den");
-      const int bx = (S.width + 255) / 256;
-      ncc_inv_den<<<bx * S.height, 256, 0, s>>>(integ, S.width, S.height, tw, th, tsums, invden, bx);
+      const int nch = den_chunks(tw);
+      if (nch <= kDenMaxChunks) {
+         const int bands = (S.height + kDenBand - 1) / kDenBand;
+         const int cblks = (S.width + kDenCols - 1) / kDenCols;
+         const size_t lds = den_lds_bytes(tw);
+         auto go = [&](auto kernel) {
+            if (lds > 64 * 1024)
+               hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                         "lds attribute");
+            kernel<<<bands * cblks, 64, lds, s>>>(static_cast<const char *>(S.ptr), S.step,
+                                                  S.width, S.height, tw, th, tsums, invden, bands);
+         };
+         switch (nch) {
+         case 1: go(ncc_den_src<1>); break;
+         case 2: go(ncc_den_src<2>); break;
+         case 3: go(ncc_den_src<3>); break;
+         case 4: go(ncc_den_src<4>); break;
+         case 5: go(ncc_den_src<5>); break;
+         case 6: go(ncc_den_src<6>); break;
+         case 7: go(ncc_den_src<7>); break;
+         case 8: go(ncc_den_src<8>); break;
+         default: go(ncc_den_src<9>); break;
+         }
+      } else {
+         if (!integ) fail(CL_INVALID_VALUE, "integral required for templates this wide");
+         const long long n = (long long)S.width * S.height;
+         ncc_inv_den_direct<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(integ, S.width, S.height,
+                                                                        tw, th, tsums, invden);
+      }
       launch_check("ncc_inv_den");
    }
    // staged columns start at x0 - tw/2 + 64 ub + 8 g with x0 a multiple of 64: 16-byte

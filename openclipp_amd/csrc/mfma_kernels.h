@@ -321,8 +321,7 @@ __device__ __forceinline__ void mfma_tile(const TileArgs &a, int bx, int by, uns
    }
 
    // C fragment: lane holds column n = lane % 32 of each n-tile, rows
-   // m = 8 * (i / 4) + 4 * (lane / 32) + i % 4.  Per n-tile, the 64 integral corners of the
-   // lane's 16 outputs are loaded before any of them is used or stored.
+   // m = 8 * (i / 4) + 4 * (lane / 32) + i % 4.
    const int n = lane & 31, mb = 4 * (lane >> 5);
    const int ybase = y0 + wave * 32 + mb;
    double tt = 0.0;
@@ -339,14 +338,20 @@ __device__ __forceinline__ void mfma_tile(const TileArgs &a, int bx, int by, uns
             val[i] = acc[j][i] * a.invden[(size_t)(y < H ? y : H - 1) * W + xc];
          }
       } else if (MODE != kModeCC) {
-         double ss[16];
+         // four outputs (16 integral corners) at a time keeps the epilogue within the
+         // registers of two waves per SIMD
 #pragma unroll
-         for (int i = 0; i < 16; ++i)
-            ss[i] = window_sqsum(a.integ, W, H, x, ybase + 8 * (i / 4) + (i % 4), a.tw, a.th);
+         for (int i0 = 0; i0 < 16; i0 += 4) {
+            double ss[4];
 #pragma unroll
-         for (int i = 0; i < 16; ++i) {
-            const double num = MODE == kModeNCC ? (double)acc[j][i] : ss[i] - 2.0 * (double)acc[j][i] + tt;
-            val[i] = (float)num * inv_sqrt(ss[i] * tt);
+            for (int i = i0; i < i0 + 4; ++i)
+               ss[i - i0] = window_sqsum(a.integ, W, H, x, ybase + 8 * (i / 4) + (i % 4), a.tw, a.th);
+#pragma unroll
+            for (int i = i0; i < i0 + 4; ++i) {
+               const double num = MODE == kModeNCC ? (double)acc[j][i]
+                                                   : ss[i - i0] - 2.0 * (double)acc[j][i] + tt;
+               val[i] = (float)num * inv_sqrt(ss[i - i0] * tt);
+            }
          }
       } else {
 #pragma unroll
@@ -377,9 +382,10 @@ void crosscorr_mfma(TileArgs a, int ldsp, int tiles_x, int tiles)
 
 // Persistent batch kernel: workgroups take (template, tile) items from `counter` until
 // `nitems * tiles` items are handed out.  Items are ordered template-major with the
-// templates sorted by decreasing cost, so the largest templates start first.
+// templates sorted by decreasing cost, so the largest templates start first.  Registers are
+// held to two waves per SIMD, the two workgroups per CU that the LDS allows.
 template <int P, int KC, int NT, int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void crosscorr_mfma_batch(const float *src, int W, int H, int sstep, int vec,
                           const double *integ, const BatchItem *__restrict__ items, int nitems,
                           int tiles_x, int tiles_y, unsigned *counter, int ldsp)

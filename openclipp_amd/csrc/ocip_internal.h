@@ -174,6 +174,9 @@ void launch_crosscorr_f32_anchored(hipStream_t s, const View &M, const float *ts
 void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &src,
                            const View &T, double *tsums, const double *integ, float *dst,
                            int dst_step, void *work);
+// The single-template MFMA Norm path forms its denominators from the source image; only
+// templates wider than 513 columns read the FP64 sum-of-squares integral instead.
+bool crosscorr_mfma_needs_integral(int tw);
 size_t crosscorr_mfma_work_bytes(int precision, int width, int height, int tw, int th);
 
 // Batched MFMA CrossCorr_Norm (ssd = false) or SqrDistance_Norm (ssd = true) of F32 1-channel
@@ -192,6 +195,10 @@ void launch_backward(hipStream_t s, int mode, const View &S, const View &G, cons
                      void *work, void *aux, char *dS, int dsstep, char *dT, int dtstep);
 
 // Min / max with first-index (row-major) tie break.
+// Per-channel sum (sqr = false) or sum of squares of an image into result[0..channels),
+// blocking.  partials: sum_partials_bytes of device scratch.
+size_t sum_partials_bytes(int width, int height);
+void launch_sum(hipStream_t s, bool sqr, const View &src, void *partials, double *result);
 size_t minmax_partials_bytes(int width, int height);
 void launch_minmax(hipStream_t s, bool want_max, const View &src, void *partials,
                    double *host_result, int *ix, int *iy);

@@ -1,4 +1,5 @@
-// stats.hip -- minimum / maximum of a 1-channel image and the pixel holding it.
+// stats.hip -- minimum / maximum of a 1-channel image and the pixel holding it; per-channel
+// sum and sum of squares.
 //
 // Reference: Statistics::Min / Max (C++/programs/Statistics.cpp:244-270) with the
 // min_coord / max_coord kernels (cl-files/Statistics.cl:342-517) and the host final reduce
@@ -7,6 +8,13 @@
 // FP64 (exact for every element type) and ties go to the first pixel in row-major order.
 // NaN pixels are skipped.  The call blocks until the result is on the host, as the
 // reference's does.
+//
+// Sum / SumSqr: reference Statistics::Sum / SumSqr (C++/programs/Statistics.cpp:350-370) with
+// reduce_sum / reduce_sum_sqr (cl-files/Statistics.cl:47-121, 279-288, 310-311) and the host
+// final reduce in double (C++/programs/StatisticsHelpers.cpp:66-76).  The reference sums each
+// work-group's 4096 pixels in REAL (float) and adds the float partials in double; here 8- and
+// 16-bit sources are summed exactly in int64 and other types in FP64, in a fixed order (the
+// result is deterministic), rounded once to double.
 #include "device_common.h"
 
 namespace ocip {
@@ -89,7 +97,112 @@ int nblocks(int W, int H)
    return (int)(b < 1 ? 1 : (b > kMaxBlocks ? kMaxBlocks : b));
 }
 
+// ---- sum / sum of squares --------------------------------------------------------------------
+
+template <int TYPE> struct SumAcc { typedef double type; };
+template <> struct SumAcc<SImage::U8>  { typedef long long type; };
+template <> struct SumAcc<SImage::S8>  { typedef long long type; };
+template <> struct SumAcc<SImage::U16> { typedef long long type; };
+template <> struct SumAcc<SImage::S16> { typedef long long type; };
+
+// One partial per block and channel: grid-stride over pixels, then wave and block sums.
+template <int TYPE, bool SQR, int NCH>
+__global__ __launch_bounds__(kThreads) void sum_partial(const char *src, int W, int H, int step,
+                                                         typename SumAcc<TYPE>::type *part)
+{
+   typedef typename SumAcc<TYPE>::type A;
+   __shared__ A red[kThreads / 64][NCH];
+   A acc[NCH];
+#pragma unroll
+   for (int c = 0; c < NCH; ++c) acc[c] = 0;
+   const long long n = (long long)W * H;
+   for (long long k = (long long)blockIdx.x * kThreads + threadIdx.x; k < n;
+        k += (long long)gridDim.x * kThreads) {
+      const int y = (int)(k / W), x = (int)(k - (long long)y * W);
+      const char *row = src + (size_t)y * step;
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+         const A v = (A)ld<TYPE>(row, x * NCH + c);
+         acc[c] += SQR ? v * v : v;
+      }
+   }
+   const int wave = threadIdx.x >> 6;
+#pragma unroll
+   for (int c = 0; c < NCH; ++c) {
+      const A w = wave_sum(acc[c]);
+      if ((threadIdx.x & 63) == 0) red[wave][c] = w;
+   }
+   __syncthreads();
+   if (threadIdx.x < NCH) {
+      A t = 0;
+      for (int w = 0; w < kThreads / 64; ++w) t += red[w][threadIdx.x];
+      part[(size_t)blockIdx.x * NCH + threadIdx.x] = t;
+   }
+}
+
+// Sum of the nparts partials of each channel into out[c] (as double), fixed order.
+template <class A, int NCH>
+__global__ __launch_bounds__(kThreads) void sum_final(const A *part, int nparts, double *out)
+{
+   __shared__ A red[kThreads / 64][NCH];
+   A acc[NCH];
+#pragma unroll
+   for (int c = 0; c < NCH; ++c) acc[c] = 0;
+   for (int k = threadIdx.x; k < nparts; k += kThreads)
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) acc[c] += part[(size_t)k * NCH + c];
+   const int wave = threadIdx.x >> 6;
+#pragma unroll
+   for (int c = 0; c < NCH; ++c) {
+      const A w = wave_sum(acc[c]);
+      if ((threadIdx.x & 63) == 0) red[wave][c] = w;
+   }
+   __syncthreads();
+   if (threadIdx.x < NCH) {
+      A t = 0;
+      for (int w = 0; w < kThreads / 64; ++w) t += red[w][threadIdx.x];
+      out[threadIdx.x] = (double)t;
+   }
+}
+
+template <int TYPE, bool SQR, int NCH>
+void launch_sum_t(hipStream_t s, const View &v, void *partials, int nb)
+{
+   typedef typename SumAcc<TYPE>::type A;
+   A *part = static_cast<A *>(partials);
+   double *out = reinterpret_cast<double *>(part + (size_t)nb * 4);
+   sum_partial<TYPE, SQR, NCH><<<nb, kThreads, 0, s>>>(static_cast<const char *>(v.ptr), v.width,
+                                                         v.height, v.step, part);
+   launch_check("sum_partial");
+   sum_final<A, NCH><<<1, kThreads, 0, s>>>(part, nb, out);
+   launch_check("sum_final");
+}
+
 }  // namespace
+
+size_t sum_partials_bytes(int W, int H) { return ((size_t)nblocks(W, H) * 4 + 4) * 8; }
+
+void launch_sum(hipStream_t s, bool sqr, const View &v, void *partials, double *result)
+{
+   const int nb = nblocks(v.width, v.height);
+   KernelTimer kt(s, "sum");
+   const int nch = v.channels;
+   if (nch != 1 && nch != 2 && nch != 4) fail(CL_INVALID_VALUE, "channels");
+   OCIP_TYPE_SWITCH(v.type, TT, {
+      if (sqr) {
+         if (nch == 1) launch_sum_t<TT, true, 1>(s, v, partials, nb);
+         else if (nch == 2) launch_sum_t<TT, true, 2>(s, v, partials, nb);
+         else launch_sum_t<TT, true, 4>(s, v, partials, nb);
+      } else {
+         if (nch == 1) launch_sum_t<TT, false, 1>(s, v, partials, nb);
+         else if (nch == 2) launch_sum_t<TT, false, 2>(s, v, partials, nb);
+         else launch_sum_t<TT, false, 4>(s, v, partials, nb);
+      }
+   });
+   const double *out = reinterpret_cast<const double *>(static_cast<char *>(partials) + (size_t)nb * 4 * 8);
+   hip_check(hipMemcpyAsync(result, out, sizeof(double) * nch, hipMemcpyDeviceToHost, s), "read");
+   hip_check(hipStreamSynchronize(s), "sync");
+}
 
 size_t minmax_partials_bytes(int W, int H)
 {

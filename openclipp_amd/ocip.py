@@ -81,6 +81,8 @@ SIGNATURES = {
                     ctypes.POINTER(ctypes.c_int)],
     "ocipMaxIndx": [_vp, _vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
                     ctypes.POINTER(ctypes.c_int)],
+    "ocipSum": [_vp, _vp, ctypes.POINTER(ctypes.c_double)],
+    "ocipSumSqr": [_vp, _vp, ctypes.POINTER(ctypes.c_double)],
     "ocipCrossCorrEx": [_vp, _vp, _vp, ctypes.c_int],
     "ocipCrossCorr_NormEx": [_vp, _vp, _vp, ctypes.c_int],
     "ocipCrossCorr_NormBatch": [_vp, _ptr_vp, _ptr_vp, ctypes.c_uint, ctypes.c_int],

@@ -156,6 +156,74 @@ def min_max_indx(src, want_max):
     return v.value, x.value, y.value
 
 
+def sum_ref(src, sqr=False):
+    """Statistics::Sum / SumSqr per channel in the reference's order, for 8-32-bit and F32
+    sources (REAL = float).  cl-files/Statistics.cl:47-121 (REDUCE): work-item (lx, ly) of a
+    16x16 work-group at (gxg, gyg) sums, in REAL, the pixels (256 gxg + lx + 16 i,
+    16 gyg + ly), i = 0..15 inside the image (the first one, then the rest in order; SQR
+    squares each in REAL, :202, :311); the 256 work-item sums (lid = 16 ly + lx) are reduced
+    by a halving tree, lid += lid + stride when both hold pixels (DO_REDUCE, :28-34);
+    :279-288 store_value keeps one float partial per group (gid = gyg * ngx + gxg);
+    C++/programs/StatisticsHelpers.cpp:66-76 adds the partials in double in gid order."""
+    a = np.asarray(src)
+    if a.ndim == 2:
+        a = a[:, :, None]
+    H, W, C = a.shape
+    assert a.dtype != np.float64
+    ngx, ngy = -(-W // 256), -(-H // 16)
+    P = np.zeros((ngy * 16, ngx * 256, C), np.float32)
+    P[:H, :W] = a.astype(np.float32)
+    if sqr:
+        P = P * P
+    valid = np.zeros((ngy * 16, ngx * 256), bool)
+    valid[:H, :W] = True
+    # column 256 gxg + 16 i + lx -> axes [gyg, ly, gxg, i, lx]
+    Q = P.reshape(ngy, 16, ngx, 16, 16, C)
+    V = valid.reshape(ngy, 16, ngx, 16, 16)
+    res = Q[:, :, :, 0].copy()
+    for i in range(1, 16):
+        res = np.where(V[:, :, :, i, :, None], res + Q[:, :, :, i], res)
+    nb = V.sum(axis=3)                                    # [gyg, ly, gxg, lx]
+    buf = res.transpose(0, 2, 1, 3, 4).reshape(ngy, ngx, 256, C)
+    cnt = nb.transpose(0, 2, 1, 3).reshape(ngy, ngx, 256)
+    stride = 128
+    while stride >= 1:
+        both = (cnt[:, :, :stride] > 0) & (cnt[:, :, stride:2 * stride] > 0)
+        buf[:, :, :stride] = np.where(both[..., None], buf[:, :, :stride] + buf[:, :, stride:2 * stride],
+                                      buf[:, :, :stride])
+        cnt[:, :, :stride] = np.where(both, cnt[:, :, :stride] + cnt[:, :, stride:2 * stride],
+                                      cnt[:, :, :stride])
+        stride //= 2
+    part = buf[:, :, 0].reshape(ngy * ngx, C).astype(np.float64)
+    out = part[0].copy()
+    for g in range(1, ngy * ngx):
+        out += part[g]
+    return out
+
+
+def sum_exact(src, sqr=False):
+    """Exact per-channel sum (or sum of squares) rounded once to double."""
+    import math
+    from fractions import Fraction
+    a = np.asarray(src)
+    if a.ndim == 2:
+        a = a[:, :, None]
+    out = []
+    for c in range(a.shape[2]):
+        v = a[:, :, c].ravel()
+        if np.issubdtype(v.dtype, np.integer):
+            x = [int(t) for t in v]
+            out.append(float(sum(t * t for t in x) if sqr else sum(x)))
+        elif sqr and v.dtype == np.float32:
+            # squares of float32 values are exact in double
+            out.append(math.fsum(float(t) * float(t) for t in v))
+        elif sqr:
+            out.append(float(sum(Fraction(float(t)) ** 2 for t in v)))
+        else:
+            out.append(math.fsum(float(t) for t in v))
+    return np.array(out)
+
+
 def f64_from_i64(op, src, tpl):
     """Exact-integer map of `op` for an integer source, as float64 (Norm in FP64)."""
     if op == SSD:

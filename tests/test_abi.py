@@ -63,6 +63,8 @@ def test_no_context_errors():
     assert L.ocipChangeContext(None) == ocip.CL_INVALID_CONTEXT
     assert L.ocipCrossCorr_Norm(None, None, None) == ocip.CL_INVALID_CONTEXT
     assert L.ocipPrepareIntegral(ctypes.byref(h), None) == ocip.CL_INVALID_CONTEXT
+    assert L.ocipSum(None, None, None) == ocip.CL_INVALID_CONTEXT
+    assert L.ocipSumSqr(None, None, None) == ocip.CL_INVALID_CONTEXT
     assert L.ocipReleaseProgram(None) == ocip.CL_INVALID_PROGRAM
     assert L.ocipSendImage(None) == ocip.CL_INVALID_MEM_OBJECT
     assert L.ocipReadImage(None) == ocip.CL_INVALID_MEM_OBJECT

@@ -1,8 +1,11 @@
 """Batched template matching (ocipCrossCorr_NormBatch, ocipSqrDistance_NormBatch).
 
-* Each map of a batch is bit-identical to the single-template call at the same precision
-  (same tile code; the per-pixel accumulation order does not depend on the tile width the
-  autotuner picks), and within the precision bound of the FP64 oracle.
+* Each map of a batch equals the single-template call at the same precision to 1e-6
+  relative: the contraction is the same tile code (the per-pixel accumulation order does not
+  depend on the tile width the autotuner picks); the single MFMA call forms 1 / sqrt(SS TT)
+  from the source image and the batch from the FP64 integral, which differ by float
+  rounding.  The F32 batch runs the single-template plain path and is bit-identical.  Every
+  map is within the precision bound of the FP64 oracle.
 * The batch is recorded into a HIP graph on its first call; later calls replay it.  Replays
   are checked one by one, with host work, new template contents sent between replays, and
   synchronisations in between.
@@ -40,6 +43,18 @@ def run_batch(ocip, fn, src, tpls, prec, S=None, Ts=None, Ds=None):
     return [d.read().copy() for d in Ds], (S, Ts, Ds)
 
 
+def same_map(batch, single, exact):
+    """Batch map against the single-template call: identical NaN positions and values equal
+    (exact) or within 1e-6 relative (the denominators' rounding)."""
+    if exact:
+        np.testing.assert_array_equal(batch, single)
+        return
+    np.testing.assert_array_equal(np.isnan(batch), np.isnan(single))
+    ok = ~np.isnan(single)
+    b, s = batch[ok].astype(np.float64), single[ok].astype(np.float64)
+    assert np.all(np.abs(b - s) <= 1e-6 * np.abs(s) + 1e-30), float(np.max(np.abs(b - s)))
+
+
 def make_case(seed, H=190, W=270, n=10, lo=3, hi=40):
     rng = np.random.default_rng(seed)
     src = rng.random((H, W), dtype=np.float32)
@@ -59,7 +74,7 @@ def test_ncc_batch_equals_single_calls(ocip, prec):
     for t, g in zip(tpls, got):
         single = run_op(ocip, "CrossCorr_Norm", src, t,
                         precision=None if prec == "f32" else PREC[prec])
-        np.testing.assert_array_equal(g, single)
+        same_map(g, single, prec == "f32")
         f64 = orc.prox_f64(orc.CC_NORM, src, t)
         kappa = integral_kappa(src, t)
         bound = (UNIT[prec] + 2 * (t.size + 16) * 2.0 ** -24 + 16 * 2.0 ** -53 * kappa) * 1.01
@@ -99,7 +114,7 @@ def test_graph_replays_track_new_inputs(ocip):
         say("replay", rep)
         got, _ = run_batch(ocip, ocip.lib.ocipCrossCorr_NormBatch, src, tpls, "bf16", S, Ts, Ds)
         for t, g in zip(tpls, got):
-            np.testing.assert_array_equal(g, run_op(ocip, "CrossCorr_Norm", src, t, precision=1))
+            same_map(g, run_op(ocip, "CrossCorr_Norm", src, t, precision=1), False)
         # host work between replays: new template contents, sent explicitly
         for T in Ts:
             T.arr[...] = rng.random(T.arr.shape, dtype=np.float32)

@@ -104,6 +104,56 @@ def test_min_max_indx(ocip, dtype):
         assert (v.value, x.value, y.value) == orc.min_max_indx(src, want_max)
 
 
+SUM_CASES = [((1, 1), 1), ((300, 517), 1), ((64, 4096), 1), ((37, 129), 2), ((20, 300), 4)]
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int8, np.uint16, np.int16, np.int32,
+                                   np.float32, np.float64])
+@pytest.mark.parametrize("shape,channels", SUM_CASES)
+def test_sum_and_sum_sqr(ocip, dtype, shape, channels):
+    """ocipSum / ocipSumSqr per channel: 8/16-bit sources bit-exact (exact int64 sums rounded
+    once to double); 32-bit and float sources within 4e-16 * n_terms relative of the exact
+    sum of |terms|, and within the float-partial bound of the reference-order sum."""
+    rng = np.random.default_rng(41)
+    full = shape + ((channels,) if channels > 1 else ())
+    if np.issubdtype(dtype, np.integer):
+        info = np.iinfo(dtype)
+        src = rng.integers(max(info.min, -30000), min(info.max, 60000) + 1, full).astype(dtype)
+    else:
+        src = (rng.random(full) - 0.3).astype(dtype)
+    S = ocip.Image(src, channels=channels)
+    say("sum", dtype.__name__, shape, channels)
+    res = (ctypes.c_double * channels)()
+    for sqr in (False, True):
+        fn = ocip.lib.ocipSumSqr if sqr else ocip.lib.ocipSum
+        ocip.check(fn(None, S.handle, res))
+        got = np.array(res[:channels])
+        want = orc.sum_exact(src, sqr)
+        if np.dtype(dtype).itemsize <= 2 and np.issubdtype(dtype, np.integer):
+            np.testing.assert_array_equal(got, want)
+            continue
+        x = src.astype(np.float64).reshape(-1, channels)
+        mag = np.abs(x * x if sqr else x).sum(axis=0)
+        assert np.all(np.abs(got - want) <= 4e-16 * 64 * mag + 1e-300), (got, want)
+        if dtype == np.float32:
+            ref = orc.sum_ref(src, sqr)
+            assert np.all(np.abs(got - ref) <= 26 * 2.0 ** -24 * mag + 1e-300)
+
+
+def test_sum_program_and_errors(ocip):
+    src = np.arange(12, dtype=np.float32).reshape(3, 4)
+    S = ocip.Image(src)
+    prog = ctypes.c_void_p()
+    ocip.check(ocip.lib.ocipPrepareStatistics(ctypes.byref(prog), S.handle))
+    r = ctypes.c_double()
+    ocip.check(ocip.lib.ocipSum(prog, S.handle, ctypes.byref(r)))
+    assert r.value == 66.0
+    ocip.check(ocip.lib.ocipSumSqr(prog, S.handle, ctypes.byref(r)))
+    assert r.value == float((np.arange(12) ** 2).sum())
+    assert ocip.lib.ocipSum(prog, S.handle, None) == ocip.CL_INVALID_VALUE
+    ocip.check(ocip.lib.ocipReleaseProgram(prog))
+
+
 # ----------------------------------------------------------------------------- ABI behaviour
 def test_context_and_images(ocip):
     name = ctypes.create_string_buffer(256)

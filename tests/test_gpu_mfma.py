@@ -68,6 +68,54 @@ def test_image_edges_and_odd_sizes(ocip, prec):
         f64 = orc.prox_f64(orc.CC, src, tpl)
         got = run_op(ocip, "CrossCorr", src, tpl, precision=PREC[prec]).astype(np.float64)
         assert np.all(np.abs(got - f64) <= bound(prec, src, tpl)), (H, W)
+        ncc64 = orc.prox_f64(orc.CC_NORM, src, tpl)
+        got = run_op(ocip, "CrossCorr_Norm", src, tpl, precision=PREC[prec]).astype(np.float64)
+        ok = np.isfinite(ncc64)
+        tol = 2 * UNIT[prec] + 2e-6 + 16 * 2.0 ** -53 * integral_kappa(src, tpl)
+        assert np.all((np.abs(got - ncc64) <= tol * (1 + np.abs(ncc64)))[ok]), (H, W)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "bf16x3"])
+def test_tall_template(ocip, prec):
+    """A template taller than one band of the denominator pass (128 output rows)."""
+    rng = np.random.default_rng(11)
+    src = rng.random((330, 70), dtype=np.float32)
+    tpl = rng.random((310, 5), dtype=np.float32)
+    f64 = orc.prox_f64(orc.CC, src, tpl)
+    got = run_op(ocip, "CrossCorr", src, tpl, precision=PREC[prec]).astype(np.float64)
+    assert np.all(np.abs(got - f64) <= bound(prec, src, tpl))
+    ncc64 = orc.prox_f64(orc.CC_NORM, src, tpl)
+    got = run_op(ocip, "CrossCorr_Norm", src, tpl, precision=PREC[prec]).astype(np.float64)
+    assert np.max(np.abs(got - ncc64)) <= 2 * UNIT[prec] + 1e-6
+
+
+@pytest.mark.parametrize("prec", ["bf16", "bf16x3"])
+def test_norm_zero_windows(ocip, prec):
+    """All-zero windows give 0/0 = NaN exactly where the FP64 evaluation does (the
+    denominators' column and row sums are differences of two prefixes of one sequence, so an
+    all-zero window sums to exactly 0), and windows overlapping the zero block stay within
+    the bound."""
+    rng = np.random.default_rng(21)
+    src = rng.random((160, 200), dtype=np.float32)
+    src[30:110, 40:150] = 0
+    tpl = rng.random((17, 24), dtype=np.float32)
+    ncc64 = orc.prox_f64(orc.CC_NORM, src, tpl)
+    got = run_op(ocip, "CrossCorr_Norm", src, tpl, precision=PREC[prec]).astype(np.float64)
+    assert np.isnan(ncc64).sum() > 1000
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(ncc64))
+    ok = np.isfinite(ncc64)
+    assert np.max(np.abs(got - ncc64)[ok]) <= 2 * UNIT[prec] + 1e-6
+
+
+def test_wide_template_reads_the_integral(ocip):
+    """Templates wider than 513 columns take the integral-based denominator pass."""
+    rng = np.random.default_rng(23)
+    src = rng.random((40, 620), dtype=np.float32)
+    tpl = rng.random((3, 530), dtype=np.float32)
+    ncc64 = orc.prox_f64(orc.CC_NORM, src, tpl)
+    got = run_op(ocip, "CrossCorr_Norm", src, tpl, precision=1).astype(np.float64)
+    tol = 2 * UNIT["bf16"] + 2e-6 + 16 * 2.0 ** -53 * integral_kappa(src, tpl)
+    assert np.all(np.abs(got - ncc64) <= tol * (1 + np.abs(ncc64)))
 
 
 def test_f32_precision_is_the_plain_path(ocip):

@@ -228,3 +228,43 @@ def test_min_max_indx_first_occurrence():
     tie = np.zeros((4, 5), np.uint8)
     tie[1, 3] = tie[2, 0] = 9
     assert orc.min_max_indx(tie, True) == (9.0, 3, 1)
+
+
+def test_sum_ref_exact_where_float_partials_are_exact():
+    """Every group partial of these images is an integer below 2^24, so the reference-order
+    sum must equal the exact integer sum: a dropped, repeated or misplaced pixel (a wrong
+    stride, group size or tree step) changes it."""
+    rng = np.random.default_rng(31)
+    for shape in [(1, 1), (16, 256), (17, 257), (70, 300), (33, 517)]:
+        a = rng.integers(0, 256, shape).astype(np.uint8)
+        assert orc.sum_ref(a)[0] == int(a.astype(np.int64).sum())
+        b = rng.integers(0, 16, shape).astype(np.uint8)
+        assert orc.sum_ref(b, True)[0] == int((b.astype(np.int64) ** 2).sum())
+    # channels are independent
+    c = rng.integers(0, 256, (20, 300, 4)).astype(np.uint8)
+    want = [int(c[:, :, k].astype(np.int64).sum()) for k in range(4)]
+    assert list(orc.sum_ref(c)) == want
+
+
+def test_sum_ref_float_within_accumulation_bound():
+    """F32 sums: within the float accumulation bound of the exact sum (16 serial + 8 tree
+    additions per group in float), and the rounding really is float (not exact)."""
+    rng = np.random.default_rng(32)
+    a = (rng.random((100, 700)) - 0.3).astype(np.float32)
+    for sqr in (False, True):
+        exact = orc.sum_exact(a, sqr)[0]
+        x = a.astype(np.float64)
+        mag = float(np.abs(x * x if sqr else x).sum())
+        got = orc.sum_ref(a, sqr)[0]
+        assert abs(got - exact) <= 26 * 2.0 ** -24 * mag
+        assert got != exact
+
+
+def test_sum_exact_against_fsum():
+    rng = np.random.default_rng(33)
+    a = rng.random((50, 60))
+    import math
+    assert orc.sum_exact(a)[0] == math.fsum(a.ravel())
+    assert orc.sum_exact(a.astype(np.float32), True)[0] == math.fsum(
+        float(t) ** 2 for t in a.astype(np.float32).ravel())
+    assert orc.sum_exact(np.array([[3, -4]], np.int16), True)[0] == 25.0
