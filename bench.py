@@ -93,6 +93,37 @@ def batch_bench(ocip, steps):
     return out
 
 
+def fft_bench(ocip, src, steps):
+    """The FFT path (ocipCrossCorrFFT_Norm, FP64 transforms) on the 4096x4096 image with the
+    64x64 template, and a 256x256 crop of the same image against the BF16 MFMA direct path
+    (the FFT's cost does not depend on the template size, the direct path's grows with it)."""
+    import ctypes
+    H, W = src.shape
+    out = {}
+    S = ocip.Image(src).send()
+    D = ocip.Image(shape=(H, W), dtype=np.float32, device_only=True)
+    for name, (th, tw) in (("t64", (64, 64)), ("t256", (256, 256))):
+        tpl = np.ascontiguousarray(src[1000:1000 + th, 2000:2000 + tw])
+        T = ocip.Image(tpl).send()
+        prog = ctypes.c_void_p()
+        ocip.check(ocip.lib.ocipPrepareImageProximityFFT(ctypes.byref(prog), S.handle, T.handle))
+        tpix = W * H * tw * th
+        fn = lambda: ocip.lib.ocipCrossCorrFFT_Norm(prog, S.handle, T.handle, D.handle)
+        ms, _ = time_steps(ocip, fn, steps, 1, "fft_cols_pair")
+        row = {"fft_value": round(tpix / (ms * 1e-3) / 1e9, 3), "fft_ms_per_step": round(ms, 3)}
+        if name == "t256":
+            fn2 = lambda: ocip.lib.ocipCrossCorr_NormEx(S.handle, T.handle, D.handle, 1)
+            ms2, _ = time_steps(ocip, fn2, max(2, steps // 2), 1, "crosscorr_mfma")
+            row.update({"mfma_bf16_value": round(tpix / (ms2 * 1e-3) / 1e9, 3),
+                        "mfma_bf16_ms_per_step": round(ms2, 3)})
+        out[name] = row
+        ocip.check(ocip.lib.ocipReleaseProgram(prog))
+        T.release()
+    S.release()
+    D.release()
+    return out
+
+
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "crosscorr_norm_4096_t64_traffic.md")
 
 
@@ -151,6 +182,7 @@ def main():
     ap.add_argument("--precision", default="bf16", choices=list(PREC_CODE))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-fft", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB device buffer (scale for rocprofv3 FETCH_SIZE/"
                          "WRITE_SIZE readings)")
@@ -229,6 +261,8 @@ def main():
     }
     if not args.no_batch:
         line["batch_2048"] = batch_bench(ocip, max(2, args.steps // 4))
+    if not args.no_fft:
+        line["fft_4096"] = fft_bench(ocip, src, max(2, args.steps // 4))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(src, tpl)
     for i in (S, T, D, D32):

@@ -139,6 +139,21 @@ ocipError ocip_API ocipMaxIndx(ocipProgram Program, ocipImage Source, double * R
 ocipError ocip_API ocipSum(ocipProgram Program, ocipImage Source, double * Result);    /* ref :552 */
 ocipError ocip_API ocipSumSqr(ocipProgram Program, ocipImage Source, double * Result); /* ref :553 */
 
+/* Proximity through FFTs (reference include/OpenCLIPP.h:673-688) -------------------------
+ * F32 1-channel images; faster than the direct ops for large templates.  The program comes
+ * from ocipPrepareImageProximityFFT (required).  The transforms are computed in FP64. */
+ocipBool  ocip_API ocipIsFFTAvailable(void);                                              /* ref :615 */
+ocipError ocip_API ocipPrepareImageProximityFFT(ocipProgram * ProgramPtr, ocipImage Image,
+                                                ocipImage Template);                     /* ref :676 */
+ocipError ocip_API ocipSqrDistanceFFT(ocipProgram Program, ocipImage Source,
+                                      ocipImage Template, ocipImage Dest);               /* ref :679 */
+ocipError ocip_API ocipSqrDistanceFFT_Norm(ocipProgram Program, ocipImage Source,
+                                           ocipImage Template, ocipImage Dest);          /* ref :682 */
+ocipError ocip_API ocipCrossCorrFFT(ocipProgram Program, ocipImage Source,
+                                    ocipImage Template, ocipImage Dest);                 /* ref :685 */
+ocipError ocip_API ocipCrossCorrFFT_Norm(ocipProgram Program, ocipImage Source,
+                                         ocipImage Template, ocipImage Dest);            /* ref :688 */
+
 /* ===================== extensions (no counterpart in the reference) ===================== */
 
 /* Arithmetic used by the cross-correlation sums. */

@@ -462,6 +462,7 @@ ocipError ocip_API ocipReleaseProgram(ocipProgram Program)
          (void)hipStreamSynchronize(p->ctx->stream);
       }
       p->scratch.release();
+      if (p->fft) fft_state_destroy(p->fft);
       p->magic = 0;
       delete p;
    });
@@ -633,6 +634,86 @@ static ocipError run_sum(ocipProgram Program, ocipImage Source, double *Result, 
       launch_sum(c.stream, sqr, view(S), buf.ptr, Result);
    });
 }
+
+// ---- proximity through FFTs ------------------------------------------------------------------
+// OpenCLIPP/OpenCLIPP.cpp:630-640: the program object owns the transform state; the image and
+// template only size it (any later call re-sizes it for its own images).
+ocipError ocip_API ocipPrepareImageProximityFFT(ocipProgram *ProgramPtr, ocipImage Image,
+                                                ocipImage Template)
+{
+   if (!g_current) return CL_INVALID_CONTEXT;
+   return guarded([&] {
+      if (!ProgramPtr) fail(CL_INVALID_VALUE, "null program pointer");
+      ocip::Program *p = new ocip::Program();
+      p->ctx = g_current;
+      p->kind = kProgProximityFFT;
+      p->fft = fft_state_create();
+      *ProgramPtr = reinterpret_cast<ocipProgram>(p);
+      if (Image && Template) {
+         ocip::Image &S = img(Image), &T = img(Template);
+         int Wb, Hb;
+         throw_if(fft_sizes(S.desc.Width, S.desc.Height, T.desc.Width, T.desc.Height, &Wb, &Hb),
+                  "image too big for the FFT path");
+         set_device(*g_current);
+         fft_prepare(g_current->stream, p->fft, S.desc.Width, S.desc.Height, T.desc.Width,
+                     T.desc.Height);
+      }
+   });
+}
+
+static ocipError run_proximity_fft(int op, ocipProgram Program, ocipImage Source,
+                                   ocipImage Template, ocipImage Dest)
+{
+   if (!g_current) return CL_INVALID_CONTEXT;
+   return guarded([&] {
+      ocip::Program &p = prog(Program, kProgProximityFFT);
+      Context &c = *p.ctx;
+      Image &S = img(Source), &T = img(Template), &D = img(Dest);
+      throw_if(check::proximity_fft(S.desc, T.desc, D.desc), "proximity FFT arguments");
+      int Wb, Hb;
+      throw_if(fft_sizes(S.desc.Width, S.desc.Height, T.desc.Width, T.desc.Height, &Wb, &Hb),
+               "image too big for the FFT path");
+      same_context(c, S); same_context(c, T); same_context(c, D);
+      set_device(c);
+      send_if_needed(c, S);
+      send_if_needed(c, T);
+      const double *integ = op == kCC ? nullptr : sqr_integral_for(c, S, 0);
+      const int tpitch = tpitch_for(T.desc.Width);
+      c.tmpl.data.ensure((size_t)tpitch * T.desc.Height * sizeof(float));
+      c.tmpl.sums.ensure(2 * sizeof(double));
+      launch_template_prep(c.stream, view(T), 0, tpitch, 0, c.tmpl.data.ptr, c.tmpl.sums.as<double>());
+      launch_proximity_fft(c.stream, p.fft, op, view(S), view(T), integ, c.tmpl.sums.as<double>(),
+                           D.dev, (int)D.desc.Step);
+      D.in_device = true;
+   });
+}
+
+ocipError ocip_API ocipSqrDistanceFFT(ocipProgram Program, ocipImage Source, ocipImage Template,
+                                      ocipImage Dest)
+{
+   return run_proximity_fft(kSSD, Program, Source, Template, Dest);
+}
+
+ocipError ocip_API ocipSqrDistanceFFT_Norm(ocipProgram Program, ocipImage Source,
+                                           ocipImage Template, ocipImage Dest)
+{
+   return run_proximity_fft(kSSDNorm, Program, Source, Template, Dest);
+}
+
+ocipError ocip_API ocipCrossCorrFFT(ocipProgram Program, ocipImage Source, ocipImage Template,
+                                    ocipImage Dest)
+{
+   return run_proximity_fft(kCC, Program, Source, Template, Dest);
+}
+
+ocipError ocip_API ocipCrossCorrFFT_Norm(ocipProgram Program, ocipImage Source,
+                                         ocipImage Template, ocipImage Dest)
+{
+   return run_proximity_fft(kCCNorm, Program, Source, Template, Dest);
+}
+
+// OpenCLIPP/OpenCLIPP.cpp:621-625: this library always has its FFT path.
+ocipBool ocip_API ocipIsFFTAvailable() { return 1; }
 
 ocipError ocip_API ocipSum(ocipProgram Program, ocipImage Source, double *Result)
 {

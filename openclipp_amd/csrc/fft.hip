@@ -1,0 +1,477 @@
+// fft.hip -- template matching through 2-D FFTs (the reference's large-template path).
+//
+// Reference: ImageProximityFFT::{CrossCorr, CrossCorr_Norm, SqrDistance, SqrDistance_Norm}
+// (C++/programs/ImageProximityFFT.cpp:115-267) and its kernels
+// (cl-files/ImageProximityFFT.cl:55-212): the source is placed at (tw/2, th/2) in a zero image
+// of Wb x Hb (W + tw/2, H + th/2 rounded up to a length whose only prime factors are 2, 3
+// and 5, ImageProximityFFT.cpp:48-56, FFT.cpp:85-103), the template at (0, 0) in another; the
+// correlation is the inverse FFT of FFT(source) * conj(FFT(template)) / (Wb Hb)
+// (mulAndScaleSpectrums), cropped to W x H; the Norm ops divide by sqrt(SS TT) with SS the
+// window sum of s^2 from the square integral (AreaSqSum) and TT = SumSqr(template).  The
+// circular correlation equals the zero-padded one because every wrapped index lands in the
+// zero border.
+//
+// This build computes in FP64 complex arithmetic (the reference uses clFFT in single
+// precision and an F32 integral) with its own mixed-radix (2, 3, 4, 5) Stockham FFT in LDS:
+//   pass 1 (fft_rows_pack): one workgroup per row y of the packed image z = s + i t (source
+//          and template share one complex transform), FFT along x, written transposed;
+//   pass 2 (fft_cols_pair): one workgroup per pair of spectrum columns (kx, -kx): FFT along y
+//          of both, the spectra of s and t separated by Hermitian symmetry
+//          (S = (Z(k) + conj Z(-k)) / 2, T = (Z(k) - conj Z(-k)) / 2i), the product
+//          conj(S conj T) / (Wb Hb), FFT along y again (an inverse transform of conjugated
+//          data), written transposed;
+//   pass 3 (fft_rows_epi): one workgroup per output row: FFT along x, real part (the
+//          conjugations cancel in it), the epilogue of the op, stored as float.
+#include <cmath>
+#include <vector>
+
+#include "device_common.h"
+
+namespace ocip {
+
+namespace {
+
+constexpr int kFftThreads = 1024;
+constexpr int kFftMaxStages = 24;
+// one row of complex doubles in the 160 KiB of LDS (less room for static LDS)
+constexpr int kFftMaxLen = (160 * 1024 - 256) / 16;
+
+struct FftPlan {
+   int n, nstages, twsize;
+   int radix[kFftMaxStages];
+   int span[kFftMaxStages];   // product of the radices of the earlier stages
+   int twoff[kFftMaxStages];  // offset of the stage's twiddles exp(-2 pi i k / (span radix))
+};
+
+// q = x / d and r = x mod d for 0 <= x < 2^24, 0 < d, from a float reciprocal corrected by
+// one step (the float quotient is within one of the true one)
+__device__ __forceinline__ int div_small(int x, int d, float rcp, int &r)
+{
+   int q = (int)((float)x * rcp);
+   r = x - q * d;
+   if (r < 0) {
+      --q;
+      r += d;
+   } else if (r >= d) {
+      ++q;
+      r -= d;
+   }
+   return q;
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// -i z and +i z
+__device__ __forceinline__ double2 mul_mi(double2 z) { return make_double2(z.y, -z.x); }
+__device__ __forceinline__ double2 mul_pi(double2 z) { return make_double2(-z.y, z.x); }
+__device__ __forceinline__ double2 cscale(double2 z, double s) { return make_double2(z.x * s, z.y * s); }
+
+// Forward DFTs of length R (sign -1) in place.
+template <int R> __device__ __forceinline__ void dft(double2 (&u)[R]);
+
+template <> __device__ __forceinline__ void dft<2>(double2 (&u)[2])
+{
+   const double2 a = u[0], b = u[1];
+   u[0] = cadd(a, b);
+   u[1] = csub(a, b);
+}
+
+template <> __device__ __forceinline__ void dft<3>(double2 (&u)[3])
+{
+   const double s = 0.86602540378443864676;  // sin(2 pi / 3)
+   const double2 a = u[0], t1 = cadd(u[1], u[2]), d = csub(u[1], u[2]);
+   const double2 t2 = make_double2(a.x - 0.5 * t1.x, a.y - 0.5 * t1.y);
+   const double2 t3 = cscale(mul_mi(d), s);
+   u[0] = cadd(a, t1);
+   u[1] = cadd(t2, t3);
+   u[2] = csub(t2, t3);
+}
+
+template <> __device__ __forceinline__ void dft<4>(double2 (&u)[4])
+{
+   const double2 t0 = cadd(u[0], u[2]), t1 = csub(u[0], u[2]);
+   const double2 t2 = cadd(u[1], u[3]), t3 = mul_mi(csub(u[1], u[3]));
+   u[0] = cadd(t0, t2);
+   u[2] = csub(t0, t2);
+   u[1] = cadd(t1, t3);
+   u[3] = csub(t1, t3);
+}
+
+template <> __device__ __forceinline__ void dft<5>(double2 (&u)[5])
+{
+   const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;  // cos(2pi/5), cos(4pi/5)
+   const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;   // sin(2pi/5), sin(4pi/5)
+   const double2 a = u[0];
+   const double2 t1 = cadd(u[1], u[4]), t2 = cadd(u[2], u[3]);
+   const double2 t3 = csub(u[1], u[4]), t4 = csub(u[2], u[3]);
+   const double2 m1 = make_double2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
+   const double2 m2 = make_double2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
+   const double2 n1 = make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+   const double2 n2 = make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+   u[0] = make_double2(a.x + t1.x + t2.x, a.y + t1.y + t2.y);
+   u[1] = cadd(m1, mul_mi(n1));
+   u[4] = cadd(m1, mul_pi(n1));
+   u[2] = cadd(m2, mul_mi(n2));
+   u[3] = cadd(m2, mul_pi(n2));
+}
+
+// Workgroup barrier that orders LDS only (a full __syncthreads also waits for the
+// workgroup's outstanding global loads and stores, which the FFT stages do not need).
+__device__ __forceinline__ void lds_barrier()
+{
+   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+   __builtin_amdgcn_s_barrier();
+   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// One Stockham stage of radix R over nseq sequences of length n stored back to back in LDS:
+// butterfly b (k = b mod span) takes x[b + j n/R], multiplies input j by w^(j k),
+// w = exp(-2 pi i / (span R)), and writes DFT output j to x[(b - k) R + k + j span].  All
+// inputs are read into registers before any output is written (one buffer).
+template <int R>
+__device__ __forceinline__ void fft_stage(double2 *x, int n, int nseq, int span,
+                                          const double2 *__restrict__ tw)
+{
+   constexpr int MAXB = (kFftMaxLen / R + kFftThreads - 1) / kFftThreads;
+   const int m = n / R, total = m * nseq;
+   const float rm = 1.0f / (float)m, rs = 1.0f / (float)span;
+   double2 u[MAXB][R];
+#pragma unroll
+   for (int q = 0; q < MAXB; ++q) {
+      const int g = threadIdx.x + q * kFftThreads;
+      if (g < total) {
+         int b, k;
+         const int seq = nseq == 1 ? (b = g, 0) : div_small(g, m, rm, b);
+         div_small(b, span, rs, k);
+         const double2 *xs = x + seq * n;
+#pragma unroll
+         for (int j = 0; j < R; ++j) u[q][j] = xs[b + j * m];
+         if (k) {
+            const double2 w = tw[k];
+            double2 wj = w;
+#pragma unroll
+            for (int j = 1; j < R; ++j) {
+               u[q][j] = cmul(u[q][j], wj);
+               if (j + 1 < R) wj = cmul(wj, w);
+            }
+         }
+         dft<R>(u[q]);
+      }
+   }
+   lds_barrier();
+#pragma unroll
+   for (int q = 0; q < MAXB; ++q) {
+      const int g = threadIdx.x + q * kFftThreads;
+      if (g < total) {
+         int b, k;
+         const int seq = nseq == 1 ? (b = g, 0) : div_small(g, m, rm, b);
+         div_small(b, span, rs, k);
+         double2 *xs = x + seq * n + (b - k) * R + k;
+#pragma unroll
+         for (int j = 0; j < R; ++j) xs[j * span] = u[q][j];
+      }
+   }
+   lds_barrier();
+}
+
+// Forward FFT (unnormalised) of nseq sequences of length P.n in LDS, natural order in and out;
+// tw: the plan's twiddles, in LDS.
+__device__ __forceinline__ void lds_fft(double2 *x, int nseq, const FftPlan *__restrict__ P, const double2 *tw)
+{
+   const int n = P->n, ns = P->nstages;
+   for (int s = 0; s < ns; ++s) {
+      const double2 *t = tw + P->twoff[s];
+      const int span = P->span[s];
+      switch (P->radix[s]) {
+      case 2: fft_stage<2>(x, n, nseq, span, t); break;
+      case 3: fft_stage<3>(x, n, nseq, span, t); break;
+      case 4: fft_stage<4>(x, n, nseq, span, t); break;
+      default: fft_stage<5>(x, n, nseq, span, t); break;
+      }
+   }
+}
+
+constexpr int kRowRegs = (kFftMaxLen + kFftThreads - 1) / kFftThreads;   // per thread, one row
+
+// The three passes are persistent: gridDim.x workgroups (one per CU, a multiple of 8) take
+// items t * gridDim.x + slot, slot = xcd_tile(blockIdx.x): at any time the workgroups of one
+// XCD work on consecutive items, so the transposed 16-byte stores of neighbouring items fill
+// the same lines of that XCD's L2.  The next item's input is loaded into registers while the
+// current one is transformed in LDS.
+
+// Pass 1: row y of z = s + i t (s at (tw/2, th/2), t at (0, 0), zero elsewhere), FFT along
+// x, stored transposed: out[kx * Hb + y].
+__global__ __launch_bounds__(kFftThreads) void fft_rows_pack(const char *__restrict__ S, int W,
+                                                             int H, int sstep,
+                                                             const char *__restrict__ T, int tw,
+                                                             int th, int tstep, int Hb,
+                                                             const FftPlan *__restrict__ pw,
+                                                             const double2 *__restrict__ tww,
+                                                             double2 *__restrict__ out)
+{
+   extern __shared__ double2 fbuf[];
+   const int Wb = pw->n, cx = tw / 2;
+   const int y = xcd_tile(blockIdx.x, gridDim.x);
+   double2 *ltw = fbuf + Wb;
+   for (int i = threadIdx.x; i < pw->twsize; i += kFftThreads) ltw[i] = tww[i];
+   const int sy = y - th / 2;
+   const bool srow = sy >= 0 && sy < H, trow = y < th;
+   const float *sr = reinterpret_cast<const float *>(S + (size_t)(srow ? sy : 0) * sstep);
+   const float *tr = reinterpret_cast<const float *>(T + (size_t)(trow ? y : 0) * tstep);
+   for (int x = threadIdx.x; x < Wb; x += kFftThreads) {
+      const int sx = x - cx;
+      const double sv = (srow && sx >= 0 && sx < W) ? (double)sr[sx] : 0.0;
+      const double tv = (trow && x < tw) ? (double)tr[x] : 0.0;
+      fbuf[x] = make_double2(sv, tv);
+   }
+   lds_barrier();
+   lds_fft(fbuf, 1, pw, ltw);
+   for (int x = threadIdx.x; x < Wb; x += kFftThreads) out[(size_t)x * Hb + y] = fbuf[x];
+}
+
+// Spectrum product at k from Z(k) = z1 and Z(-k) = z2, conjugated for the inverse transform.
+__device__ __forceinline__ double2 spectrum_product(double2 z1, double2 z2, double scale)
+{
+   const double2 s = cscale(cadd(z1, cconj(z2)), 0.5);           // FFT of the source
+   const double2 t = cscale(mul_mi(csub(z1, cconj(z2))), 0.5);   // FFT of the template
+   return cconj(cscale(cmul(s, cconj(t)), scale));
+}
+
+// Pass 2: spectrum columns kx and Wb - kx (rows of the transposed array), kx = 0..Wb/2: FFT
+// along y, product, FFT along y again; stored transposed (rows y < H only): out[y * Wb + kx].
+__global__ __launch_bounds__(kFftThreads) void fft_cols_pair(const double2 *__restrict__ in, int Wb,
+                                                             int H, const FftPlan *__restrict__ ph,
+                                                             const double2 *__restrict__ twh,
+                                                             double scale,
+                                                             double2 *__restrict__ out)
+{
+   extern __shared__ double2 fbuf[];
+   const int Hb = ph->n;
+   double2 *ltw = fbuf + 2 * Hb;
+   for (int i = threadIdx.x; i < ph->twsize; i += kFftThreads) ltw[i] = twh[i];
+   {
+      const int kx = xcd_tile(blockIdx.x, gridDim.x);
+      const int kx2 = (Wb - kx) % Wb;
+      const int nseq = kx2 == kx ? 1 : 2;
+      double2 *A = fbuf, *B = nseq == 2 ? fbuf + Hb : fbuf;
+      for (int y = threadIdx.x; y < Hb; y += kFftThreads) {
+         A[y] = in[(size_t)kx * Hb + y];
+         if (nseq == 2) B[y] = in[(size_t)kx2 * Hb + y];
+      }
+      lds_barrier();
+      lds_fft(fbuf, nseq, ph, ltw);
+      // the four entries of a pair (ky, -ky) of both columns are read and written by one thread
+      for (int ky = threadIdx.x; ky <= Hb / 2; ky += kFftThreads) {
+         const int ky2 = ky == 0 ? 0 : Hb - ky;
+         const double2 a1 = A[ky], a2 = A[ky2], b1 = B[ky], b2 = B[ky2];
+         A[ky] = spectrum_product(a1, b2, scale);
+         A[ky2] = spectrum_product(a2, b1, scale);
+         if (nseq == 2) {
+            B[ky] = spectrum_product(b1, a2, scale);
+            B[ky2] = spectrum_product(b2, a1, scale);
+         }
+      }
+      lds_barrier();
+      lds_fft(fbuf, nseq, ph, ltw);
+      for (int y = threadIdx.x; y < H; y += kFftThreads) {
+         out[(size_t)y * Wb + kx] = A[y];
+         if (nseq == 2) out[(size_t)y * Wb + kx2] = B[y];
+      }
+   }
+}
+
+// Pass 3: output row y < H: FFT along x, real part = the correlation, the op's epilogue
+// (ImageProximityFFT.cl:86-147, in FP64 with SS from the FP64 integral).
+__global__ __launch_bounds__(kFftThreads) void fft_rows_epi(const double2 *__restrict__ in, int W,
+                                                            int H, const FftPlan *__restrict__ pw,
+                                                            const double2 *__restrict__ tww,
+                                                            int op, int tw, int th,
+                                                            const double *__restrict__ integ,
+                                                            const double *__restrict__ tsums,
+                                                            char *__restrict__ dst, int dstep)
+{
+   extern __shared__ double2 fbuf[];
+   const int Wb = pw->n;
+   double2 *ltw = fbuf + Wb;
+   for (int i = threadIdx.x; i < pw->twsize; i += kFftThreads) ltw[i] = tww[i];
+   const double tt = op == kCC ? 0.0 : tsums[0];
+   {
+      const int y = blockIdx.x;
+      for (int x = threadIdx.x; x < Wb; x += kFftThreads) fbuf[x] = in[(size_t)y * Wb + x];
+      lds_barrier();
+      lds_fft(fbuf, 1, pw, ltw);
+      float *drow = reinterpret_cast<float *>(dst + (size_t)y * dstep);
+      for (int x = threadIdx.x; x < W; x += kFftThreads) {
+         const double cc = fbuf[x].x;
+         double v = cc;
+         if (op != kCC) {
+            const double ss = window_sqsum(integ, W, H, x, y, tw, th);
+            if (op == kCCNorm) v = cc / sqrt(ss * tt);
+            else if (op == kSSD) v = ss - 2.0 * cc + tt;
+            else v = (ss - 2.0 * cc + tt) / sqrt(ss * tt);
+         }
+         drow[x] = (float)v;
+      }
+   }
+}
+
+// Stages for a 2^a 3^b 5^c length, larger radices last (the twiddle table of a stage has span
+// entries, so this keeps the table small); twiddles appended to `table`, offsets relative to
+// the plan's first entry.
+void make_plan(int n, FftPlan &p, std::vector<double2> &table)
+{
+   p.n = n;
+   p.nstages = 0;
+   const int base = (int)table.size();
+   int rest = n, span = 1;
+   auto add = [&](int r) {
+      if (p.nstages >= kFftMaxStages) fail(CL_INVALID_IMAGE_SIZE, "FFT length");
+      p.radix[p.nstages] = r;
+      p.span[p.nstages] = span;
+      p.twoff[p.nstages] = (int)table.size() - base;
+      for (int k = 0; k < span; ++k) {
+         const long double a = -2.0L * 3.14159265358979323846264338327950288L * k / ((long double)span * r);
+         table.push_back(make_double2((double)cosl(a), (double)sinl(a)));
+      }
+      span *= r;
+      rest /= r;
+      ++p.nstages;
+   };
+   int twos = 0, m = n;
+   while (m % 2 == 0) {
+      m /= 2;
+      ++twos;
+   }
+   if (twos % 2) add(2);
+   while (rest % 3 == 0) add(3);
+   while (rest % 4 == 0) add(4);
+   while (rest % 5 == 0) add(5);
+   if (rest != 1) fail(CL_INVALID_IMAGE_SIZE, "FFT length with a prime factor above 5");
+   p.twsize = (int)table.size() - base;
+}
+
+// LDS of the three passes: data plus the plan's twiddles, in complex doubles
+size_t row_lds(const FftPlan &p) { return (size_t)(p.n + p.twsize) * sizeof(double2); }
+size_t pair_lds(const FftPlan &p) { return (size_t)(2 * p.n + p.twsize) * sizeof(double2); }
+
+}  // namespace
+
+// FFT.cpp:85-103 IsSupportedLength (multiples of 5 accepted: not an NVIDIA platform)
+bool fft_supported_length(int n)
+{
+   if (n < 1) return false;
+   for (int f : {2, 3, 5})
+      while (n % f == 0) n /= f;
+   return n == 1;
+}
+
+// ImageProximityFFT.cpp:48-56
+int fft_padded_length(int n)
+{
+   while (!fft_supported_length(n)) ++n;
+   return n;
+}
+
+struct FftState {
+   int Wb = 0, Hb = 0, hbase = 0;   // hbase: first twiddle of the height plan
+   FftPlan pw{}, ph{};
+   DevBuf twid, plans, a, b;   // plans: pw and ph, read by the kernels
+   ~FftState()
+   {
+      twid.release();
+      plans.release();
+      a.release();
+      b.release();
+   }
+};
+
+FftState *fft_state_create() { return new FftState(); }
+void fft_state_destroy(FftState *s) { delete s; }
+
+cl_int fft_sizes(int W, int H, int tw, int th, int *Wb, int *Hb)
+{
+   *Wb = fft_padded_length(W + tw / 2);
+   *Hb = fft_padded_length(H + th / 2);
+   // The reference stops at 2^24 padded points (a clFFT limit, FFT.cpp:170-171); here the
+   // limit is LDS: one row of the padded width, or two columns of the padded height, of
+   // complex doubles plus the plan's twiddles per workgroup.
+   if (*Wb > kFftMaxLen || 2 * *Hb > kFftMaxLen) return CL_INVALID_IMAGE_SIZE;
+   std::vector<double2> table;
+   FftPlan pw, ph;
+   make_plan(*Wb, pw, table);
+   make_plan(*Hb, ph, table);
+   if (row_lds(pw) > (size_t)kFftMaxLen * sizeof(double2) ||
+       pair_lds(ph) > (size_t)kFftMaxLen * sizeof(double2))
+      return CL_INVALID_IMAGE_SIZE;
+   return CL_SUCCESS;
+}
+
+// Sizes the state's plans and buffers for a W x H source and tw x th template.
+void fft_prepare(hipStream_t s, FftState *st, int W, int H, int tw, int th)
+{
+   int Wb, Hb;
+   if (fft_sizes(W, H, tw, th, &Wb, &Hb) != CL_SUCCESS) fail(CL_INVALID_IMAGE_SIZE, "image too big for the FFT path");
+   if (st->Wb == Wb && st->Hb == Hb) return;
+   std::vector<double2> table;  // the width plan's twiddles, then the height plan's
+   make_plan(Wb, st->pw, table);
+   st->hbase = (int)table.size();
+   make_plan(Hb, st->ph, table);
+   st->twid.ensure(table.size() * sizeof(double2));
+   hip_check(hipMemcpyAsync(st->twid.ptr, table.data(), table.size() * sizeof(double2),
+                            hipMemcpyHostToDevice, s), "twiddles");
+   const FftPlan both[2] = {st->pw, st->ph};
+   st->plans.ensure(sizeof(both));
+   hip_check(hipMemcpyAsync(st->plans.ptr, both, sizeof(both), hipMemcpyHostToDevice, s), "plans");
+   hip_check(hipStreamSynchronize(s), "twiddles");
+   st->a.ensure((size_t)Wb * Hb * sizeof(double2));
+   st->b.ensure((size_t)Wb * Hb * sizeof(double2));
+   st->Wb = Wb;
+   st->Hb = Hb;
+   static bool lds_set = false;
+   if (!lds_set) {
+      for (const void *k : {reinterpret_cast<const void *>(fft_rows_pack),
+                            reinterpret_cast<const void *>(fft_cols_pair),
+                            reinterpret_cast<const void *>(fft_rows_epi)})
+         hip_check(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       kFftMaxLen * (int)sizeof(double2)),
+                   "lds attribute");
+      lds_set = true;
+   }
+}
+
+void launch_proximity_fft(hipStream_t s, FftState *st, int op, const View &S, const View &T,
+                          const double *integ, const double *tsums, void *dst, int dstep)
+{
+   fft_prepare(s, st, S.width, S.height, T.width, T.height);
+   const int Wb = st->Wb, Hb = st->Hb;
+   const double2 *tww = static_cast<const double2 *>(st->twid.ptr), *twh = tww + st->hbase;
+   double2 *a = static_cast<double2 *>(st->a.ptr), *b = static_cast<double2 *>(st->b.ptr);
+   const FftPlan *plans = static_cast<const FftPlan *>(st->plans.ptr);
+   {
+      KernelTimer kt(s, "fft_rows_pack");
+      fft_rows_pack<<<Hb, kFftThreads, row_lds(st->pw), s>>>(
+          static_cast<const char *>(S.ptr), S.width, S.height, S.step,
+          static_cast<const char *>(T.ptr), T.width, T.height, T.step, Hb, plans, tww, a);
+      launch_check("fft_rows_pack");
+   }
+   {
+      KernelTimer kt(s, "fft_cols_pair");
+      fft_cols_pair<<<Wb / 2 + 1, kFftThreads, pair_lds(st->ph), s>>>(
+          a, Wb, S.height, plans + 1, twh, 1.0 / ((double)Wb * Hb), b);
+      launch_check("fft_cols_pair");
+   }
+   {
+      KernelTimer kt(s, "fft_rows_epi");
+      fft_rows_epi<<<S.height, kFftThreads, row_lds(st->pw), s>>>(
+          b, S.width, S.height, plans, tww, op, T.width, T.height, integ, tsums,
+          static_cast<char *>(dst), dstep);
+      launch_check("fft_rows_epi");
+   }
+}
+
+}  // namespace ocip

@@ -67,13 +67,18 @@ struct Image {
    bool in_device = false;
 };
 
-enum ProgramKind { kProgIntegral = 1, kProgStatistics = 2 };
+enum ProgramKind { kProgIntegral = 1, kProgStatistics = 2, kProgProximityFFT = 3 };
+
+struct FftState;
+FftState *fft_state_create();
+void fft_state_destroy(FftState *s);
 
 struct Program {
    uint32_t magic = kProgramMagic;
    Context *ctx = nullptr;
    int kind = 0;
    DevBuf scratch;
+   FftState *fft = nullptr;   // kProgProximityFFT: plans, twiddles and transform buffers
 };
 
 // Per-template data the proximity kernels read: the template as F32 (or F64 / int32)
@@ -195,6 +200,15 @@ void launch_backward(hipStream_t s, int mode, const View &S, const View &G, cons
                      void *work, void *aux, char *dS, int dsstep, char *dT, int dtstep);
 
 // Min / max with first-index (row-major) tie break.
+// Proximity through 2-D FFTs (FP64): op kSSD, kSSDNorm, kCC or kCCNorm on an F32 1-channel
+// source and template into an F32 dest.  integ: the exclusive FP64 sum-of-squares integral
+// (all ops but kCC); tsums[0] = sum t^2.  fft_sizes gives the padded transform size or
+// CL_INVALID_IMAGE_SIZE when the image is too big for the path.
+cl_int fft_sizes(int W, int H, int tw, int th, int *Wb, int *Hb);
+void fft_prepare(hipStream_t s, FftState *st, int W, int H, int tw, int th);
+void launch_proximity_fft(hipStream_t s, FftState *st, int op, const View &S, const View &T,
+                          const double *integ, const double *tsums, void *dst, int dstep);
+
 // Per-channel sum (sqr = false) or sum of squares of an image into result[0..channels),
 // blocking.  partials: sum_partials_bytes of device scratch.
 size_t sum_partials_bytes(int width, int height);

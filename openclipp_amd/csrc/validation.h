@@ -59,6 +59,22 @@ inline cl_int proximity(const SImage &S, const SImage &T, const SImage &D)
    return CL_SUCCESS;
 }
 
+// C++/programs/ImageProximityFFT.cpp:117-130 (each of the four ops): CheckFloat(D),
+// CheckSameNbChannels(S, T) and (S, D), CheckSameSize(S, D), template <= source
+// (CL_IMAGE_FORMAT_NOT_SUPPORTED), SameType(S, T) (CL_IMAGE_FORMAT_MISMATCH); then the
+// contract of include/OpenCLIPP.h:678-688, "Images must be F32 - 1 channel".
+inline cl_int proximity_fft(const SImage &S, const SImage &T, const SImage &D)
+{
+   if (!floating(D.Type)) return CL_INVALID_VALUE;
+   if (S.Channels != T.Channels || S.Channels != D.Channels) return CL_INVALID_VALUE;
+   if (S.Width != D.Width || S.Height != D.Height) return CL_INVALID_IMAGE_SIZE;
+   if (T.Width > S.Width || T.Height > S.Height) return CL_IMAGE_FORMAT_NOT_SUPPORTED;
+   if (!same_type(S, T)) return -9 /* CL_IMAGE_FORMAT_MISMATCH */;
+   if (S.Type != SImage::F32 || S.Channels != 1 || D.Type != SImage::F32)
+      return CL_IMAGE_FORMAT_NOT_SUPPORTED;
+   return CL_SUCCESS;
+}
+
 // C++/programs/Integral.cpp:97-146: Dest F32/F64, then CheckSameSize.  One channel on both
 // sides (include/OpenCLIPP.h:642-645; the reference does not check it).
 inline cl_int integral(const SImage &S, const SImage &D)

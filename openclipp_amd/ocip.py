@@ -83,6 +83,12 @@ SIGNATURES = {
                     ctypes.POINTER(ctypes.c_int)],
     "ocipSum": [_vp, _vp, ctypes.POINTER(ctypes.c_double)],
     "ocipSumSqr": [_vp, _vp, ctypes.POINTER(ctypes.c_double)],
+    "ocipIsFFTAvailable": [],
+    "ocipPrepareImageProximityFFT": [_ptr_vp, _vp, _vp],
+    "ocipSqrDistanceFFT": [_vp, _vp, _vp, _vp],
+    "ocipSqrDistanceFFT_Norm": [_vp, _vp, _vp, _vp],
+    "ocipCrossCorrFFT": [_vp, _vp, _vp, _vp],
+    "ocipCrossCorrFFT_Norm": [_vp, _vp, _vp, _vp],
     "ocipCrossCorrEx": [_vp, _vp, _vp, ctypes.c_int],
     "ocipCrossCorr_NormEx": [_vp, _vp, _vp, ctypes.c_int],
     "ocipCrossCorr_NormBatch": [_vp, _ptr_vp, _ptr_vp, ctypes.c_uint, ctypes.c_int],
@@ -101,7 +107,8 @@ for _name, _args in SIGNATURES.items():
     _f = getattr(lib, _name)
     _f.argtypes = _args
     _f.restype = ctypes.c_char_p if _name == "ocipGetErrorName" else (
-        None if _name == "ocipSetCLFilesPath" else ctypes.c_int)
+        None if _name == "ocipSetCLFilesPath" else (
+            ctypes.c_char if _name == "ocipIsFFTAvailable" else ctypes.c_int))
 
 
 class OcipError(RuntimeError):

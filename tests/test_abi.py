@@ -64,6 +64,9 @@ def test_no_context_errors():
     assert L.ocipCrossCorr_Norm(None, None, None) == ocip.CL_INVALID_CONTEXT
     assert L.ocipPrepareIntegral(ctypes.byref(h), None) == ocip.CL_INVALID_CONTEXT
     assert L.ocipSum(None, None, None) == ocip.CL_INVALID_CONTEXT
+    assert L.ocipCrossCorrFFT_Norm(None, None, None, None) == ocip.CL_INVALID_CONTEXT
+    assert L.ocipPrepareImageProximityFFT(ctypes.byref(h), None, None) == ocip.CL_INVALID_CONTEXT
+    assert L.ocipIsFFTAvailable() == b"\x01"
     assert L.ocipSumSqr(None, None, None) == ocip.CL_INVALID_CONTEXT
     assert L.ocipReleaseProgram(None) == ocip.CL_INVALID_PROGRAM
     assert L.ocipSendImage(None) == ocip.CL_INVALID_MEM_OBJECT
