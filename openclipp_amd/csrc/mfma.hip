@@ -58,7 +58,11 @@ __device__ float kZeroPad = 0.f;   // never written: the zero read for positions
 int den_chunks(int tw) { return (kDenCols - 1 + tw + 63) / 64; }
 size_t den_lds_bytes(int tw) { return (size_t)kDenRows * (den_chunks(tw) * 64 + 1) * sizeof(float); }
 
-template <int NCH>
+// Offset that puts a buffer load out of range (the hardware returns 0): used for positions
+// outside the image when the image holds fewer than kDenOob bytes.
+constexpr int kDenOob = 0x40000000;
+
+template <int NCH, bool BUF>
 __global__ __launch_bounds__(64) void ncc_den_src(const char *__restrict__ src, int sstep, int W,
                                                   int H, int tw, int th,
                                                   const double *__restrict__ tsums,
@@ -78,22 +82,26 @@ __global__ __launch_bounds__(64) void ncc_den_src(const char *__restrict__ src, 
    const int x0 = cblk * kDenCols, cx = tw / 2, cy = th / 2;
    const int y0 = band * kDenBand, n = min(kDenBand, H - y0);
    const double tt = tsums[0];
-   int col[NCH];
+   int col[NCH], coff[NCH];
    bool cok[NCH];
 #pragma unroll
    for (int k = 0; k < NCH; ++k) {
       const int c = x0 - cx + 64 * k + lane;
       cok[k] = c >= 0 && c < W && 64 * k + lane < kDenCols - 1 + tw;
       col[k] = min(max(c, 0), W - 1);
+      coff[k] = cok[k] ? c * 4 : kDenOob;
    }
+   const __amdgpu_buffer_rsrc_t rsrc =
+       __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(src), (short)0, BUF ? H * sstep : 0, 0x00020000);
    const int pre = th - 1;                       // steps before the first output row
    const int jstart = -((U - pre % U) % U);      // (pre - jstart) % U == 0
    const int b0 = (pre - jstart) / U;            // first batch that completes output rows
    const int nb = b0 + (n + U - 1) / U;
 
-   // A position outside the image (or a step that takes no row) reads the zero of kZeroPad:
-   // every load is unconditional (an address select, no branch around it), so a whole batch
-   // is in flight at once.
+   // Every load is unconditional, so a whole batch is in flight at once.  BUF: a buffer load
+   // whose offset (row + column, either kDenOob when outside) is out of range returns 0.
+   // Otherwise a position outside the image (or a step that takes no row) reads the zero of
+   // kZeroPad (an address select, no branch around the load).
    auto load_batch = [&](int bb, float (&L)[U][NCH], float (&G)[U][NCH]) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -102,12 +110,22 @@ __global__ __launch_bounds__(64) void ncc_den_src(const char *__restrict__ src, 
          const int lr = y0 - cy + j, gr = lr - th;
          const bool lok = live && j >= 0 && lr >= 0 && lr < H;
          const bool gok = live && j >= th && gr >= 0 && gr < H;
-         const float *lrow = reinterpret_cast<const float *>(src + (size_t)min(max(lr, 0), H - 1) * sstep);
-         const float *grow = reinterpret_cast<const float *>(src + (size_t)min(max(gr, 0), H - 1) * sstep);
+         if (BUF) {
+            const int loff = lok ? lr * sstep : kDenOob, goff = gok ? gr * sstep : kDenOob;
 #pragma unroll
-         for (int k = 0; k < NCH; ++k) {
-            L[u][k] = *((lok && cok[k]) ? lrow + col[k] : &kZeroPad);
-            G[u][k] = *((gok && cok[k]) ? grow + col[k] : &kZeroPad);
+            for (int k = 0; k < NCH; ++k) {
+               const unsigned lo = (unsigned)coff[k] + (unsigned)loff, go = (unsigned)coff[k] + (unsigned)goff;
+               L[u][k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)lo, 0, 0));
+               G[u][k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)go, 0, 0));
+            }
+         } else {
+            const float *lrow = reinterpret_cast<const float *>(src + (size_t)min(max(lr, 0), H - 1) * sstep);
+            const float *grow = reinterpret_cast<const float *>(src + (size_t)min(max(gr, 0), H - 1) * sstep);
+#pragma unroll
+            for (int k = 0; k < NCH; ++k) {
+               L[u][k] = *((lok && cok[k]) ? lrow + col[k] : &kZeroPad);
+               G[u][k] = *((gok && cok[k]) ? grow + col[k] : &kZeroPad);
+            }
          }
       }
    };
@@ -368,17 +386,23 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
             kernel<<<bands * cblks, 64, lds, s>>>(static_cast<const char *>(S.ptr), S.step,
                                                   S.width, S.height, tw, th, tsums, invden, bands);
          };
+         // buffer loads when every byte offset, and two out-of-range offsets added, stay
+         // below 2^31
+         const bool buf = (long long)S.height * S.step <= kDenOob;
+#define OCIP_DEN_CASE(N) \
+   case N: if (buf) go(ncc_den_src<N, true>); else go(ncc_den_src<N, false>); break;
          switch (nch) {
-         case 1: go(ncc_den_src<1>); break;
-         case 2: go(ncc_den_src<2>); break;
-         case 3: go(ncc_den_src<3>); break;
-         case 4: go(ncc_den_src<4>); break;
-         case 5: go(ncc_den_src<5>); break;
-         case 6: go(ncc_den_src<6>); break;
-         case 7: go(ncc_den_src<7>); break;
-         case 8: go(ncc_den_src<8>); break;
-         default: go(ncc_den_src<9>); break;
+         OCIP_DEN_CASE(1)
+         OCIP_DEN_CASE(2)
+         OCIP_DEN_CASE(3)
+         OCIP_DEN_CASE(4)
+         OCIP_DEN_CASE(5)
+         OCIP_DEN_CASE(6)
+         OCIP_DEN_CASE(7)
+         OCIP_DEN_CASE(8)
+         default: if (buf) go(ncc_den_src<9, true>); else go(ncc_den_src<9, false>); break;
          }
+#undef OCIP_DEN_CASE
       } else {
          if (!integ) fail(CL_INVALID_VALUE, "integral required for templates this wide");
          const long long n = (long long)S.width * S.height;

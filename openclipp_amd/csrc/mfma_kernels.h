@@ -210,17 +210,28 @@ __device__ __forceinline__ void row_mfma(f32x16 (&acc)[NT], const unsigned short
                                          size_t plane, const u16x8 (&b0)[KC],
                                          const u16x8 (&b1)[KC])
 {
+   if constexpr (NPL == 1) {
+      // chunk-major: consecutive MFMAs go to different accumulators; each accumulator still
+      // takes its chunks in the same order
+      constexpr int NA = 2 * (NT - 1) + KC;
+      u16x8 a0[NA];
 #pragma unroll
-   for (int c = 0; c < 2 * (NT - 1) + KC; ++c) {
-      const u16x8 a0 = *reinterpret_cast<const u16x8 *>(ab + 16 * c);
-      u16x8 a1;
-      if (NPL == 2) a1 = *reinterpret_cast<const u16x8 *>(ab + plane + 16 * c);
+      for (int c = 0; c < NA; ++c) a0[c] = *reinterpret_cast<const u16x8 *>(ab + 16 * c);
 #pragma unroll
-      for (int j = 0; j < NT; ++j) {
-         const int kc = c - 2 * j;
-         if (kc < 0 || kc >= KC) continue;
-         acc[j] = mma<P>(a0, b0[kc], acc[j]);
-         if (NPL == 2) {
+      for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+         for (int j = 0; j < NT; ++j) acc[j] = mma<P>(a0[2 * j + kc], b0[kc], acc[j]);
+   } else {
+      // split BF16 (three products per chunk, registers for one A column pair at a time)
+#pragma unroll
+      for (int c = 0; c < 2 * (NT - 1) + KC; ++c) {
+         const u16x8 a0 = *reinterpret_cast<const u16x8 *>(ab + 16 * c);
+         const u16x8 a1 = *reinterpret_cast<const u16x8 *>(ab + plane + 16 * c);
+#pragma unroll
+         for (int j = 0; j < NT; ++j) {
+            const int kc = c - 2 * j;
+            if (kc < 0 || kc >= KC) continue;
+            acc[j] = mma<P>(a0, b0[kc], acc[j]);
             acc[j] = mma<P>(a0, b1[kc], acc[j]);
             acc[j] = mma<P>(a1, b0[kc], acc[j]);
          }
@@ -371,7 +382,8 @@ __device__ __forceinline__ void mfma_tile(const TileArgs &a, int bx, int by, uns
 // may spend registers on keeping two template rows of B fragments in flight.
 // One-dimensional grid of tiles_x * tiles_y workgroups in XCD-aware order.
 template <int P, int KC, int NT, int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(NT == 2 ? 3 : 1, NT == 2 ? 3 : 2)))
 void crosscorr_mfma(TileArgs a, int ldsp, int tiles_x, int tiles)
 {
    extern __shared__ u16x8 lds_v[];

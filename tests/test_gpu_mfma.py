@@ -107,6 +107,21 @@ def test_norm_zero_windows(ocip, prec):
     assert np.max(np.abs(got - ncc64)[ok]) <= 2 * UNIT[prec] + 1e-6
 
 
+def test_norm_of_an_image_over_one_gib(ocip):
+    """A pitched image whose rows span more than 2^30 bytes takes the denominator pass's
+    address-select loads instead of its buffer loads; both give the same maps."""
+    rng = np.random.default_rng(29)
+    big = np.zeros((4100, 65536), np.float32)     # 1.07 GB of rows, 64 columns used
+    src = big[:, :64]
+    src[...] = rng.random((4100, 64), dtype=np.float32)
+    tpl = rng.random((9, 11), dtype=np.float32)
+    ncc64 = orc.prox_f64(orc.CC_NORM, np.ascontiguousarray(src), tpl)
+    got = run_op(ocip, "CrossCorr_Norm", src, tpl, precision=1).astype(np.float64)
+    assert np.max(np.abs(got - ncc64)) <= 2 * UNIT["bf16"] + 1e-6
+    small = run_op(ocip, "CrossCorr_Norm", np.ascontiguousarray(src), tpl, precision=1)
+    np.testing.assert_allclose(got, small, rtol=1e-6, atol=0)
+
+
 def test_wide_template_reads_the_integral(ocip):
     """Templates wider than 513 columns take the integral-based denominator pass."""
     rng = np.random.default_rng(23)
