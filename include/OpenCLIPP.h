@@ -139,6 +139,18 @@ ocipError ocip_API ocipMaxIndx(ocipProgram Program, ocipImage Source, double * R
 ocipError ocip_API ocipSum(ocipProgram Program, ocipImage Source, double * Result);    /* ref :552 */
 ocipError ocip_API ocipSumSqr(ocipProgram Program, ocipImage Source, double * Result); /* ref :553 */
 
+/* FFT (reference include/OpenCLIPP.h:612-634) --------------------------------------------
+ * 2-D transform of a 1-channel F32 (or F64) real image of 2-3-5-smooth width and height into
+ * its half spectrum: a complex image (real, imaginary interleaved, same data type) of at least
+ * Width/2+1 complex values per row and the same height; the inverse goes back.  Neither
+ * direction is normalised (inverse(forward(x)) = Width*Height*x).  Computed in FP64. */
+ocipError ocip_API ocipPrepareFFT(ocipProgram * ProgramPtr, ocipImage RealImage,
+                                  ocipImage ComplexImage);                                /* ref :613 */
+ocipError ocip_API ocipFFTForward(ocipProgram Program, ocipImage RealSource,
+                                  ocipImage ComplexDest);                                 /* ref :625 */
+ocipError ocip_API ocipFFTInverse(ocipProgram Program, ocipImage ComplexSource,
+                                  ocipImage RealDest);                                    /* ref :634 */
+
 /* Proximity through FFTs (reference include/OpenCLIPP.h:673-688) -------------------------
  * F32 1-channel images; faster than the direct ops for large templates.  The program comes
  * from ocipPrepareImageProximityFFT (required).  The transforms are computed in FP64. */

@@ -712,6 +712,62 @@ ocipError ocip_API ocipCrossCorrFFT_Norm(ocipProgram Program, ocipImage Source,
    return run_proximity_fft(kCCNorm, Program, Source, Template, Dest);
 }
 
+// ---- generic FFT -----------------------------------------------------------------------------
+// OpenCLIPP/OpenCLIPP.cpp:606-619, C++/programs/FFT.cpp:140-283.
+static void check_fft(const Image &R, const Image &C)
+{
+   throw_if(check::fft(R.desc, C.desc), "FFT image types");
+   throw_if(fft_plain_sizes(R.desc.Width, R.desc.Height), "FFT size");
+   throw_if(check::fft_layout(R.desc, C.desc), "FFT complex image layout");
+}
+
+ocipError ocip_API ocipPrepareFFT(ocipProgram *ProgramPtr, ocipImage RealImage,
+                                  ocipImage ComplexImage)
+{
+   if (!g_current) return CL_INVALID_CONTEXT;
+   return guarded([&] {
+      if (!ProgramPtr) fail(CL_INVALID_VALUE, "null program pointer");
+      ocip::Program *p = new ocip::Program();
+      p->ctx = g_current;
+      p->kind = kProgFFT;
+      p->fft = fft_state_create();
+      *ProgramPtr = reinterpret_cast<ocipProgram>(p);
+      if (RealImage && ComplexImage) {
+         ocip::Image &R = img(RealImage), &C = img(ComplexImage);
+         check_fft(R, C);
+         set_device(*g_current);
+         fft_plain_prepare(g_current->stream, p->fft, R.desc.Width, R.desc.Height);
+      }
+   });
+}
+
+static ocipError run_fft(ocipProgram Program, ocipImage Source, ocipImage Dest, bool forward)
+{
+   if (!g_current) return CL_INVALID_CONTEXT;
+   return guarded([&] {
+      ocip::Program &p = prog(Program, kProgFFT);
+      Context &c = *p.ctx;
+      Image &S = img(Source), &D = img(Dest);
+      Image &R = forward ? S : D, &C = forward ? D : S;
+      check_fft(R, C);
+      same_context(c, S); same_context(c, D);
+      set_device(c);
+      send_if_needed(c, S);
+      launch_fft(c.stream, p.fft, forward, view(R), view(C));
+      D.in_device = true;
+   });
+}
+
+ocipError ocip_API ocipFFTForward(ocipProgram Program, ocipImage RealSource, ocipImage ComplexDest)
+{
+   return run_fft(Program, RealSource, ComplexDest, true);
+}
+
+ocipError ocip_API ocipFFTInverse(ocipProgram Program, ocipImage ComplexSource, ocipImage RealDest)
+{
+   return run_fft(Program, ComplexSource, RealDest, false);
+}
+
 // OpenCLIPP/OpenCLIPP.cpp:621-625: this library always has its FFT path.
 ocipBool ocip_API ocipIsFFTAvailable() { return 1; }
 

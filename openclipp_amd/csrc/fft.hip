@@ -320,6 +320,92 @@ __global__ __launch_bounds__(kFftThreads) void fft_rows_epi(const double2 *__res
    }
 }
 
+// ---- the generic 2-D FFT (reference FFT::Forward / Inverse, C++/programs/FFT.cpp:140-283):
+// real W x H image <-> Hermitian half spectrum (W/2 + 1) x H, complex values interleaved
+// (re, im) in F32 or F64, unnormalised both ways (clfftSetPlanScale(..., 1)).  Computed in FP64;
+// work holds (W/2 + 1) x H complex doubles, one row per spectrum column kx.
+
+// Forward, pass 1: row y of the real image, FFT along x, columns kx <= W/2 stored transposed.
+template <class T>
+__global__ __launch_bounds__(kFftThreads) void fft_r2c_rows(const char *__restrict__ src, int sstep,
+                                                            int H, const FftPlan *__restrict__ pw,
+                                                            const double2 *__restrict__ tww,
+                                                            double2 *__restrict__ work)
+{
+   extern __shared__ double2 fbuf[];
+   const int W = pw->n, y = xcd_tile(blockIdx.x, gridDim.x);
+   double2 *ltw = fbuf + W;
+   for (int i = threadIdx.x; i < pw->twsize; i += kFftThreads) ltw[i] = tww[i];
+   const T *row = reinterpret_cast<const T *>(src + (size_t)y * sstep);
+   for (int x = threadIdx.x; x < W; x += kFftThreads) fbuf[x] = make_double2((double)row[x], 0.0);
+   lds_barrier();
+   lds_fft(fbuf, 1, pw, ltw);
+   for (int kx = threadIdx.x; kx <= W / 2; kx += kFftThreads) work[(size_t)kx * H + y] = fbuf[kx];
+}
+
+// Forward, pass 2: spectrum column kx, FFT along y, stored as row ky, column kx of dst.
+template <class T>
+__global__ __launch_bounds__(kFftThreads) void fft_r2c_cols(const double2 *__restrict__ work,
+                                                            const FftPlan *__restrict__ ph,
+                                                            const double2 *__restrict__ twh,
+                                                            char *__restrict__ dst, int dstep)
+{
+   extern __shared__ double2 fbuf[];
+   const int H = ph->n, kx = xcd_tile(blockIdx.x, gridDim.x);
+   double2 *ltw = fbuf + H;
+   for (int i = threadIdx.x; i < ph->twsize; i += kFftThreads) ltw[i] = twh[i];
+   for (int y = threadIdx.x; y < H; y += kFftThreads) fbuf[y] = work[(size_t)kx * H + y];
+   lds_barrier();
+   lds_fft(fbuf, 1, ph, ltw);
+   for (int ky = threadIdx.x; ky < H; ky += kFftThreads) {
+      T *d = reinterpret_cast<T *>(dst + (size_t)ky * dstep) + 2 * kx;
+      d[0] = (T)fbuf[ky].x;
+      d[1] = (T)fbuf[ky].y;
+   }
+}
+
+// Inverse, pass 1: column kx of the half spectrum, inverse FFT along y as the FFT of the
+// conjugate; the conjugated result stored as row kx of work.
+template <class T>
+__global__ __launch_bounds__(kFftThreads) void fft_c2r_cols(const char *__restrict__ src, int sstep,
+                                                            const FftPlan *__restrict__ ph,
+                                                            const double2 *__restrict__ twh,
+                                                            double2 *__restrict__ work)
+{
+   extern __shared__ double2 fbuf[];
+   const int H = ph->n, kx = xcd_tile(blockIdx.x, gridDim.x);
+   double2 *ltw = fbuf + H;
+   for (int i = threadIdx.x; i < ph->twsize; i += kFftThreads) ltw[i] = twh[i];
+   for (int ky = threadIdx.x; ky < H; ky += kFftThreads) {
+      const T *v = reinterpret_cast<const T *>(src + (size_t)ky * sstep) + 2 * kx;
+      fbuf[ky] = make_double2((double)v[0], -(double)v[1]);
+   }
+   lds_barrier();
+   lds_fft(fbuf, 1, ph, ltw);
+   for (int y = threadIdx.x; y < H; y += kFftThreads) work[(size_t)kx * H + y] = fbuf[y];
+}
+
+// Inverse, pass 2: row y: the conjugated row spectrum completed by Hermitian symmetry
+// (Z(W - kx) = conj Z(kx)), FFT along x, real part stored.
+template <class T>
+__global__ __launch_bounds__(kFftThreads) void fft_c2r_rows(const double2 *__restrict__ work, int H,
+                                                            const FftPlan *__restrict__ pw,
+                                                            const double2 *__restrict__ tww,
+                                                            char *__restrict__ dst, int dstep)
+{
+   extern __shared__ double2 fbuf[];
+   const int W = pw->n, y = blockIdx.x;
+   double2 *ltw = fbuf + W;
+   for (int i = threadIdx.x; i < pw->twsize; i += kFftThreads) ltw[i] = tww[i];
+   for (int kx = threadIdx.x; kx < W; kx += kFftThreads)
+      fbuf[kx] = kx <= W / 2 ? work[(size_t)kx * H + y] : cconj(work[(size_t)(W - kx) * H + y]);
+   lds_barrier();
+   lds_fft(fbuf, 1, pw, ltw);
+   T *row = reinterpret_cast<T *>(dst + (size_t)y * dstep);
+   for (int x = threadIdx.x; x < W; x += kFftThreads) row[x] = (T)fbuf[x].x;
+}
+
+
 // Stages for a 2^a 3^b 5^c length, larger radices last (the twiddle table of a stage has span
 // entries, so this keeps the table small); twiddles appended to `table`, offsets relative to
 // the plan's first entry.
@@ -471,6 +557,104 @@ void launch_proximity_fft(hipStream_t s, FftState *st, int op, const View &S, co
           b, S.width, S.height, plans, tww, op, T.width, T.height, integ, tsums,
           static_cast<char *>(dst), dstep);
       launch_check("fft_rows_epi");
+   }
+}
+
+// ---- generic FFT entry points -------------------------------------------------------------
+
+cl_int fft_plain_sizes(int W, int H)
+{
+   if (!fft_supported_length(W) || !fft_supported_length(H)) return CL_INVALID_IMAGE_SIZE;
+   std::vector<double2> table;
+   FftPlan pw, ph;
+   make_plan(W, pw, table);
+   make_plan(H, ph, table);
+   if (row_lds(pw) > (size_t)kFftMaxLen * sizeof(double2) ||
+       row_lds(ph) > (size_t)kFftMaxLen * sizeof(double2))
+      return CL_INVALID_IMAGE_SIZE;
+   return CL_SUCCESS;
+}
+
+// Plans for exactly W x H (no padding) and a (W/2 + 1) x H work buffer.
+static void fft_prepare_plain(hipStream_t s, FftState *st, int W, int H)
+{
+   if (fft_plain_sizes(W, H) != CL_SUCCESS) fail(CL_INVALID_IMAGE_SIZE, "FFT size");
+   if (st->Wb == W && st->Hb == H) return;
+   std::vector<double2> table;
+   make_plan(W, st->pw, table);
+   st->hbase = (int)table.size();
+   make_plan(H, st->ph, table);
+   st->twid.ensure(table.size() * sizeof(double2));
+   hip_check(hipMemcpyAsync(st->twid.ptr, table.data(), table.size() * sizeof(double2),
+                            hipMemcpyHostToDevice, s), "twiddles");
+   const FftPlan both[2] = {st->pw, st->ph};
+   st->plans.ensure(sizeof(both));
+   hip_check(hipMemcpyAsync(st->plans.ptr, both, sizeof(both), hipMemcpyHostToDevice, s), "plans");
+   hip_check(hipStreamSynchronize(s), "twiddles");
+   st->a.ensure((size_t)(W / 2 + 1) * H * sizeof(double2));
+   st->Wb = W;
+   st->Hb = H;
+   static bool lds_set = false;
+   if (!lds_set) {
+      for (const void *k : {reinterpret_cast<const void *>(fft_r2c_rows<float>),
+                            reinterpret_cast<const void *>(fft_r2c_rows<double>),
+                            reinterpret_cast<const void *>(fft_r2c_cols<float>),
+                            reinterpret_cast<const void *>(fft_r2c_cols<double>),
+                            reinterpret_cast<const void *>(fft_c2r_cols<float>),
+                            reinterpret_cast<const void *>(fft_c2r_cols<double>),
+                            reinterpret_cast<const void *>(fft_c2r_rows<float>),
+                            reinterpret_cast<const void *>(fft_c2r_rows<double>)})
+         hip_check(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       kFftMaxLen * (int)sizeof(double2)),
+                   "lds attribute");
+      lds_set = true;
+   }
+}
+
+void fft_plain_prepare(hipStream_t s, FftState *st, int W, int H) { fft_prepare_plain(s, st, W, H); }
+
+template <class T>
+static void fft_forward_t(hipStream_t s, FftState *st, const View &real, const View &cplx)
+{
+   const int W = real.width, H = real.height;
+   const double2 *tww = static_cast<const double2 *>(st->twid.ptr), *twh = tww + st->hbase;
+   const FftPlan *plans = static_cast<const FftPlan *>(st->plans.ptr);
+   double2 *work = static_cast<double2 *>(st->a.ptr);
+   KernelTimer kt(s, "fft_r2c");
+   fft_r2c_rows<T><<<H, kFftThreads, row_lds(st->pw), s>>>(static_cast<const char *>(real.ptr),
+                                                           real.step, H, plans, tww, work);
+   launch_check("fft_r2c_rows");
+   fft_r2c_cols<T><<<W / 2 + 1, kFftThreads, row_lds(st->ph), s>>>(
+       work, plans + 1, twh, const_cast<char *>(static_cast<const char *>(cplx.ptr)), cplx.step);
+   launch_check("fft_r2c_cols");
+}
+
+template <class T>
+static void fft_inverse_t(hipStream_t s, FftState *st, const View &cplx, const View &real)
+{
+   const int W = real.width, H = real.height;
+   const double2 *tww = static_cast<const double2 *>(st->twid.ptr), *twh = tww + st->hbase;
+   const FftPlan *plans = static_cast<const FftPlan *>(st->plans.ptr);
+   double2 *work = static_cast<double2 *>(st->a.ptr);
+   KernelTimer kt(s, "fft_c2r");
+   fft_c2r_cols<T><<<W / 2 + 1, kFftThreads, row_lds(st->ph), s>>>(
+       static_cast<const char *>(cplx.ptr), cplx.step, plans + 1, twh, work);
+   launch_check("fft_c2r_cols");
+   fft_c2r_rows<T><<<H, kFftThreads, row_lds(st->pw), s>>>(
+       work, H, plans, tww, const_cast<char *>(static_cast<const char *>(real.ptr)), real.step);
+   launch_check("fft_c2r_rows");
+}
+
+void launch_fft(hipStream_t s, FftState *st, bool forward, const View &real, const View &cplx)
+{
+   fft_prepare_plain(s, st, real.width, real.height);
+   const bool f64 = real.type == SImage::F64;
+   if (forward) {
+      if (f64) fft_forward_t<double>(s, st, real, cplx);
+      else fft_forward_t<float>(s, st, real, cplx);
+   } else {
+      if (f64) fft_inverse_t<double>(s, st, cplx, real);
+      else fft_inverse_t<float>(s, st, cplx, real);
    }
 }
 

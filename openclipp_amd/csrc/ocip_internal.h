@@ -67,7 +67,7 @@ struct Image {
    bool in_device = false;
 };
 
-enum ProgramKind { kProgIntegral = 1, kProgStatistics = 2, kProgProximityFFT = 3 };
+enum ProgramKind { kProgIntegral = 1, kProgStatistics = 2, kProgProximityFFT = 3, kProgFFT = 4 };
 
 struct FftState;
 FftState *fft_state_create();
@@ -78,7 +78,7 @@ struct Program {
    Context *ctx = nullptr;
    int kind = 0;
    DevBuf scratch;
-   FftState *fft = nullptr;   // kProgProximityFFT: plans, twiddles and transform buffers
+   FftState *fft = nullptr;   // kProgProximityFFT, kProgFFT: plans, twiddles, work buffers
 };
 
 // Per-template data the proximity kernels read: the template as F32 (or F64 / int32)
@@ -208,6 +208,13 @@ cl_int fft_sizes(int W, int H, int tw, int th, int *Wb, int *Hb);
 void fft_prepare(hipStream_t s, FftState *st, int W, int H, int tw, int th);
 void launch_proximity_fft(hipStream_t s, FftState *st, int op, const View &S, const View &T,
                           const double *integ, const double *tsums, void *dst, int dstep);
+
+// Generic 2-D FFT (F32 or F64): forward real W x H -> half spectrum (W/2 + 1) x H of
+// interleaved complex values, inverse back, unnormalised.  fft_plain_sizes: CL_SUCCESS or
+// CL_INVALID_IMAGE_SIZE (a length with a prime factor above 5, or beyond LDS).
+cl_int fft_plain_sizes(int W, int H);
+void fft_plain_prepare(hipStream_t s, FftState *st, int W, int H);
+void launch_fft(hipStream_t s, FftState *st, bool forward, const View &real, const View &cplx);
 
 // Per-channel sum (sqr = false) or sum of squares of an image into result[0..channels),
 // blocking.  partials: sum_partials_bytes of device scratch.

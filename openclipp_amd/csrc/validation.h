@@ -75,6 +75,25 @@ inline cl_int proximity_fft(const SImage &S, const SImage &T, const SImage &D)
    return CL_SUCCESS;
 }
 
+// C++/programs/FFT.cpp:140-168 (PrepareFor(Real, Complex)): same data type, F32 or F64, a
+// 1-channel real image, 2-3-5-smooth width and height (CL_INVALID_IMAGE_SIZE, checked by the
+// caller with the plan), room for W/2 + 1 complex values per complex row; then this build's
+// check that the complex image has the real image's height (the reference would go out of
+// bounds).
+inline cl_int fft(const SImage &R, const SImage &C)
+{
+   if (R.Type != C.Type) return CL_IMAGE_FORMAT_NOT_SUPPORTED;
+   if (C.Type != SImage::F32 && C.Type != SImage::F64) return CL_IMAGE_FORMAT_NOT_SUPPORTED;
+   if (R.Channels != 1) return CL_IMAGE_FORMAT_NOT_SUPPORTED;
+   return CL_SUCCESS;
+}
+inline cl_int fft_layout(const SImage &R, const SImage &C)
+{
+   if (C.Step / elem_bytes(C.Type) / 2 < R.Width / 2 + 1) return CL_IMAGE_FORMAT_NOT_SUPPORTED;
+   if (C.Height < R.Height) return CL_INVALID_IMAGE_SIZE;
+   return CL_SUCCESS;
+}
+
 // C++/programs/Integral.cpp:97-146: Dest F32/F64, then CheckSameSize.  One channel on both
 // sides (include/OpenCLIPP.h:642-645; the reference does not check it).
 inline cl_int integral(const SImage &S, const SImage &D)

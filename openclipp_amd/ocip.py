@@ -84,6 +84,9 @@ SIGNATURES = {
     "ocipSum": [_vp, _vp, ctypes.POINTER(ctypes.c_double)],
     "ocipSumSqr": [_vp, _vp, ctypes.POINTER(ctypes.c_double)],
     "ocipIsFFTAvailable": [],
+    "ocipPrepareFFT": [_ptr_vp, _vp, _vp],
+    "ocipFFTForward": [_vp, _vp, _vp],
+    "ocipFFTInverse": [_vp, _vp, _vp],
     "ocipPrepareImageProximityFFT": [_ptr_vp, _vp, _vp],
     "ocipSqrDistanceFFT": [_vp, _vp, _vp, _vp],
     "ocipSqrDistanceFFT_Norm": [_vp, _vp, _vp, _vp],

@@ -160,3 +160,76 @@ def test_fft_1024_t128(ocip):
     win = np.stack([P[y:y + 128, x:x + 128] for x, y in zip(xs, ys)])
     want = (win * t).sum(axis=(1, 2)) / np.sqrt((win * win).sum(axis=(1, 2)) * (t * t).sum())
     assert np.max(np.abs(got[ys, xs] - want)) < 1e-6
+
+
+# ------------------------------------------------------------- generic FFT (ocipFFTForward/Inverse)
+def complex_image(ocip, H, W, dtype, extra=0):
+    """A complex image for a W-wide real image: W/2 + 1 (+ extra) complex values per row, as a
+    2-channel image of the real type (real, imaginary interleaved)."""
+    return ocip.Image(np.zeros((H, W // 2 + 1 + extra, 2), dtype))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(1, 1), (8, 12), (30, 45), (64, 100), (27, 250), (81, 2)])
+def test_fft_forward_inverse_against_numpy(ocip, dtype, shape):
+    """Forward = numpy.fft.rfft2 (unnormalised half spectrum along x), inverse =
+    W*H*irfft2; FP64 inside, so F64 agrees to ~1e-13 of the sum of |x| and F32 to its own
+    rounding of the result."""
+    H, W = shape
+    rng = np.random.default_rng(H * 100 + W)
+    x = (rng.random((H, W)) - 0.4).astype(dtype)
+    R, C = ocip.Image(x), complex_image(ocip, H, W, dtype)
+    prog = ctypes.c_void_p()
+    ocip.check(ocip.lib.ocipPrepareFFT(ctypes.byref(prog), R.handle, C.handle))
+    ocip.check(ocip.lib.ocipFFTForward(prog, R.handle, C.handle))
+    got = C.read()
+    want = np.fft.rfft2(x.astype(np.float64))
+    scale = np.abs(x.astype(np.float64)).sum()
+    rel = 2.0 ** -23 if dtype == np.float32 else 0.0
+    err = np.abs(got[..., 0] + 1j * got[..., 1] - want)
+    assert np.all(err <= rel * np.abs(want) + 1e-13 * scale + 1e-300), float(err.max())
+    # inverse of the exact spectrum, written to a fresh real image
+    spec = np.zeros(got.shape, dtype)
+    spec[..., 0], spec[..., 1] = want.real, want.imag     # rounded to the image type
+    C2, R2 = ocip.Image(spec), ocip.Image(np.zeros((H, W), dtype))
+    ocip.check(ocip.lib.ocipFFTInverse(prog, C2.handle, R2.handle))
+    back = R2.read().astype(np.float64)
+    ref = np.fft.irfft2(spec[..., 0].astype(np.float64) + 1j * spec[..., 1].astype(np.float64),
+                        s=(H, W)) * (W * H)
+    tol = (2.0 ** -23 if dtype == np.float32 else 0.0) * np.abs(ref) + 1e-12 * scale * 8
+    assert np.all(np.abs(back - ref) <= tol + 1e-300), float(np.abs(back - ref).max())
+    ocip.check(ocip.lib.ocipReleaseProgram(prog))
+
+
+def test_fft_round_trip_and_padded_complex_rows(ocip):
+    """inverse(forward(x)) = W*H*x, through a complex image with room for more than W/2 + 1
+    values per row."""
+    rng = np.random.default_rng(77)
+    H, W = 96, 160
+    x = rng.random((H, W))
+    R, C, R2 = ocip.Image(x), complex_image(ocip, H, W, np.float64, extra=7), ocip.Image(np.zeros((H, W)))
+    prog = ctypes.c_void_p()
+    ocip.check(ocip.lib.ocipPrepareFFT(ctypes.byref(prog), None, None))
+    ocip.check(ocip.lib.ocipFFTForward(prog, R.handle, C.handle))
+    ocip.check(ocip.lib.ocipFFTInverse(prog, C.handle, R2.handle))
+    np.testing.assert_allclose(R2.read() / (W * H), x, rtol=0, atol=1e-13)
+    ocip.check(ocip.lib.ocipReleaseProgram(prog))
+
+
+def test_fft_argument_errors_generic(ocip):
+    L = ocip.lib
+    prog = ctypes.c_void_p()
+    ocip.check(L.ocipPrepareFFT(ctypes.byref(prog), None, None))
+    R = ocip.Image(np.zeros((8, 12), np.float32))
+    C = complex_image(ocip, 8, 12, np.float32)
+    assert L.ocipFFTForward(None, R.handle, C.handle) == ocip.CL_INVALID_PROGRAM
+    # FFT.cpp:140-168 order: same type, F32/F64, 1-channel real, 2-3-5 lengths, complex width
+    C64 = complex_image(ocip, 8, 12, np.float64)
+    assert L.ocipFFTForward(prog, R.handle, C64.handle) == ocip.CL_IMAGE_FORMAT_NOT_SUPPORTED
+    R8, C8 = ocip.Image(np.zeros((8, 12), np.uint8)), ocip.Image(np.zeros((8, 14), np.uint8))
+    assert L.ocipFFTForward(prog, R8.handle, C8.handle) == ocip.CL_IMAGE_FORMAT_NOT_SUPPORTED
+    R7 = ocip.Image(np.zeros((8, 7), np.float32))
+    assert L.ocipFFTForward(prog, R7.handle, C.handle) == ocip.CL_INVALID_IMAGE_SIZE
+    Cn = ocip.Image(np.zeros((8, 6, 2), np.float32))    # 6 < 12/2 + 1 complex values per row
+    assert L.ocipFFTForward(prog, R.handle, Cn.handle) == ocip.CL_IMAGE_FORMAT_NOT_SUPPORTED
+    ocip.check(L.ocipReleaseProgram(prog))
