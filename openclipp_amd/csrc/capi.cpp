@@ -81,13 +81,33 @@ View view(const Image &i)
                (int)i.desc.Channels, (int)i.desc.Type};
 }
 
+// Host -> device copy of an image; colour images go through their 3-channel staging copy
+// (ColorImage::Send, C++/Image.cpp:262-275).
+void upload(Context &c, Image &i)
+{
+   void *to = i.stage ? i.stage : i.dev;
+   hip_check(hipMemcpyAsync(to, i.host, i.bytes, hipMemcpyHostToDevice, c.stream), "send");
+   if (i.stage)
+      launch_copy3to4(c.stream, i.desc.Type, i.stage, i.host_desc.Step, i.dev, i.desc.Step,
+                      i.desc.Width, i.desc.Height);
+}
+
+// Device -> host copy, blocking (ColorImage::Read, C++/Image.cpp:251-259).
+void download(Context &c, Image &i)
+{
+   if (i.stage)
+      launch_copy4to3(c.stream, i.desc.Type, i.dev, i.desc.Step, i.stage, i.host_desc.Step,
+                      i.desc.Width, i.desc.Height);
+   hip_check(hipMemcpyAsync(i.host, i.stage ? i.stage : i.dev, i.bytes, hipMemcpyDeviceToHost,
+                            c.stream), "read");
+   hip_check(hipStreamSynchronize(c.stream), "read sync");
+}
+
 // Lazy host->device transfer of an input (reference kernel_helpers.h:135, Buffer.cpp:190-194)
 void send_if_needed(Context &c, Image &i)
 {
    if (i.in_device) return;
-   if (i.host)
-      hip_check(hipMemcpyAsync(i.dev, i.host, i.bytes, hipMemcpyHostToDevice, c.stream),
-                "send");
+   if (i.host) upload(c, i);
    i.in_device = true;
 }
 
@@ -403,9 +423,22 @@ ocipError ocip_API ocipCreateImage(ocipImage *ImagePtr, SImage Image, void *Imag
       ocip::Image *i = new ocip::Image();
       i->ctx = g_current;
       i->desc = Image;
+      i->host_desc = Image;
       i->host = ImageData;
       i->bytes = bytes;
-      if (hipMalloc(&i->dev, bytes) != hipSuccess) {
+      size_t dev_bytes = bytes;
+      if (Image.Channels == 3) {
+         // OpenCLIPP.cpp:185-188: a ColorImage, 4 channels on the device (Image.cpp:243-249)
+         i->desc.Channels = 4;
+         i->desc.Step = Image.Width * 4 * (uint)check::elem_bytes(Image.Type);
+         dev_bytes = (size_t)Image.Height * i->desc.Step;
+         if (hipMalloc(&i->stage, bytes) != hipSuccess) {
+            delete i;
+            fail(CL_MEM_OBJECT_ALLOCATION_FAILURE, "image allocation");
+         }
+      }
+      if (hipMalloc(&i->dev, dev_bytes) != hipSuccess) {
+         if (i->stage) (void)hipFree(i->stage);
          delete i;
          fail(CL_MEM_OBJECT_ALLOCATION_FAILURE, "image allocation");
       }
@@ -419,8 +452,7 @@ ocipError ocip_API ocipSendImage(ocipImage Image)
       ocip::Image &i = img(Image);
       if (!i.host) return;  // device-only image (C++/Buffer.cpp:158-159)
       set_device(*i.ctx);
-      hip_check(hipMemcpyAsync(i.dev, i.host, i.bytes, hipMemcpyHostToDevice, i.ctx->stream),
-                "send");
+      upload(*i.ctx, i);
       i.in_device = true;
    });
 }
@@ -431,9 +463,7 @@ ocipError ocip_API ocipReadImage(ocipImage Image)
       ocip::Image &i = img(Image);
       if (!i.host) return;  // device-only image (C++/Buffer.cpp:125-126)
       set_device(*i.ctx);
-      hip_check(hipMemcpyAsync(i.host, i.dev, i.bytes, hipMemcpyDeviceToHost, i.ctx->stream),
-                "read");
-      hip_check(hipStreamSynchronize(i.ctx->stream), "read sync");
+      download(*i.ctx, i);
    });
 }
 
@@ -447,6 +477,7 @@ ocipError ocip_API ocipReleaseImage(ocipImage Image)
          (void)hipStreamSynchronize(i.ctx->stream);
       }
       (void)hipFree(i.dev);
+      if (i.stage) (void)hipFree(i.stage);
       i.magic = 0;
       delete &i;
    });

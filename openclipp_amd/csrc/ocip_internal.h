@@ -57,15 +57,27 @@ struct BatchCache;
 enum : uint32_t { kImageMagic = 0x474d494fu /*OIMG*/, kContextMagic = 0x5854434fu /*OCTX*/,
                   kProgramMagic = 0x4752504fu /*OPRG*/ };
 
+// desc describes the device image the kernels see.  A 3-channel (colour) image is held on
+// the device as 4 channels with the fourth set to 255 (reference ColorImage,
+// C++/Image.cpp:243-275): host_desc keeps the caller's 3-channel layout, `stage` a device copy
+// of it, and transfers convert between the two.
 struct Image {
    uint32_t magic = kImageMagic;
    Context *ctx = nullptr;
    SImage desc{};
+   SImage host_desc{};
    void *host = nullptr;
    void *dev = nullptr;
-   size_t bytes = 0;
+   void *stage = nullptr;   // colour images only
+   size_t bytes = 0;        // bytes of the host image (Height * host Step)
    bool in_device = false;
 };
+
+// Colour conversions between the 3-channel host layout and the 4-channel device layout.
+void launch_copy3to4(hipStream_t s, int type, const void *src, int sstep, void *dst, int dstep,
+                     int width, int height);
+void launch_copy4to3(hipStream_t s, int type, const void *src, int sstep, void *dst, int dstep,
+                     int width, int height);
 
 enum ProgramKind { kProgIntegral = 1, kProgStatistics = 2, kProgProximityFFT = 3, kProgFFT = 4 };
 

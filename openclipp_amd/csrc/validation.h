@@ -21,12 +21,12 @@ inline int elem_bytes(int t)
 inline bool floating(int t) { return t == SImage::F32 || t == SImage::F64; }
 inline bool unsigned_(int t) { return t == SImage::U8 || t == SImage::U16 || t == SImage::U32; }
 
-// C++/Image.cpp:147-165 (channels 1..4; 3 is a separate colour class in the reference and is
-// not supported here; type in range), plus the buffer checks of this build: a non-empty
-// buffer, a Step that holds a row and is a multiple of the element size.
+// C++/Image.cpp:147-165 (channels 1..4, 3 being a colour image held as 4 channels on the
+// device, OpenCLIPP/OpenCLIPP.cpp:185-188; type in range), plus the buffer checks of this
+// build: a non-empty buffer, a Step that holds a row and is a multiple of the element size.
 inline cl_int create_image(const SImage &i)
 {
-   if (i.Channels < 1 || i.Channels > 4 || i.Channels == 3) return CL_IMAGE_FORMAT_NOT_SUPPORTED;
+   if (i.Channels < 1 || i.Channels > 4) return CL_IMAGE_FORMAT_NOT_SUPPORTED;
    if ((int)i.Type < 0 || (int)i.Type >= (int)SImage::NbDataTypes) return CL_IMAGE_FORMAT_NOT_SUPPORTED;
    if ((unsigned long long)i.Height * i.Step == 0) return -61 /* CL_INVALID_BUFFER_SIZE */;
    if ((unsigned long long)i.Step < (unsigned long long)i.Width * i.Channels * elem_bytes(i.Type))

@@ -166,7 +166,7 @@ def test_context_and_images(ocip):
     ocip.check(ocip.lib.ocipSendImage(D.handle))
     ocip.check(ocip.lib.ocipReadImage(D.handle))
     h = ctypes.c_void_p()
-    assert ocip.lib.ocipCreateImage(ctypes.byref(h), ocip.SImage(4, 4, 16, 3, ocip.F32), None,
+    assert ocip.lib.ocipCreateImage(ctypes.byref(h), ocip.SImage(4, 4, 16, 5, ocip.F32), None,
                                     ocip.CL_MEM_READ_WRITE) == ocip.CL_IMAGE_FORMAT_NOT_SUPPORTED
 
 

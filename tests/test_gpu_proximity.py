@@ -11,6 +11,8 @@ Bars:
   least 1e-6 of the image's energy, and everywhere within the float32 accumulation bound
   plus the integral's conditioning term of the FP64 oracle (see norm_close).
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -145,6 +147,40 @@ def test_multichannel(ocip, nch, dtype):
     for op in NORM_OPS:
         got = run_op(ocip, op, src, tpl).astype(np.float64)
         np.testing.assert_allclose(got, orc.prox_f64(OP_CODE[op], src, tpl), rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.uint8])
+def test_colour_images(ocip, dtype):
+    """3-channel images are colour images: 4 channels on the device with the fourth set to 255
+    (reference ColorImage), converted on send and read.  Each channel of a map equals the
+    1-channel map of that channel; the host sees 3 channels; a read restores the host data."""
+    rng = np.random.default_rng(5)
+    src = rand(rng, (29, 37, 3), dtype)
+    tpl = rand(rng, (5, 6, 3), dtype)
+    for op in RAW_OPS + NORM_OPS:
+        got = run_op(ocip, op, src, tpl)
+        assert got.shape == (29, 37, 3)
+        for c in range(3):
+            s1, t1 = np.ascontiguousarray(src[:, :, c]), np.ascontiguousarray(tpl[:, :, c])
+            if op in NORM_OPS:
+                np.testing.assert_allclose(got[:, :, c], orc.prox_f64(OP_CODE[op], s1, t1),
+                                           rtol=1e-5, atol=1e-6, err_msg=op)
+            elif dtype == np.uint8:
+                want = orc.f64_from_i64(OP_CODE[op], s1, t1).astype(np.float32)
+                np.testing.assert_array_equal(got[:, :, c], want, err_msg=op)
+            else:
+                np.testing.assert_array_equal(got[:, :, c], orc.prox_ref(OP_CODE[op], s1, t1),
+                                              err_msg=op)
+    # send, clear the host copy, read back
+    img = ocip.Image(src.copy())
+    img.send()
+    img.arr[...] = 0
+    np.testing.assert_array_equal(img.read(), src)
+    # the device image has 4 channels: Sum reports the fourth (255 per pixel) as well
+    res = (ctypes.c_double * 4)()
+    ocip.check(ocip.lib.ocipSum(None, img.handle, res))
+    assert list(res)[:3] == [float(src[:, :, c].astype(np.float64).sum()) for c in range(3)]
+    assert res[3] == 255.0 * 29 * 37
 
 
 def test_pitched_images(ocip):
