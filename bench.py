@@ -13,7 +13,7 @@ reported.
 
 Roofline: the dominant kernel's algorithmic FLOPs (2 per template pixel per output pixel)
 divided by its average duration, timed with HIP events on the library's own stream
-(ocipEnableKernelTiming) over the timed region.  Its `traffic` is the HBM bytes per launch
+(ocipEnableKernelTiming) over a second run of the timed steps.  Its `traffic` is the HBM bytes per launch
 of that kernel from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) on this command,
 calibrated on a device copy of known size (--copy-calibration), kept in
 profiles/crosscorr_norm_4096_t64_traffic.md.
@@ -55,15 +55,22 @@ def ncc_f64_at(src, tpl, xs, ys):
 
 
 def time_steps(ocip, fn, steps, warmup, kernel):
+    """ms per step over `steps` calls (wall clock between two ocipFinish), then the average
+    duration of `kernel` from HIP events recorded on the library's stream around each launch
+    over a second run of the same steps (kept out of the first run: the event records sit
+    between the kernels of a step)."""
     for _ in range(warmup):
         ocip.check(fn())
     ocip.check(ocip.lib.ocipFinish())
-    ocip.check(ocip.lib.ocipEnableKernelTiming(b"\x01"))
     t0 = time.perf_counter()
     for _ in range(steps):
         ocip.check(fn())
     ocip.check(ocip.lib.ocipFinish())
     t1 = time.perf_counter()
+    ocip.check(ocip.lib.ocipEnableKernelTiming(b"\x01"))
+    for _ in range(steps):
+        ocip.check(fn())
+    ocip.check(ocip.lib.ocipFinish())
     kms, kn = ocip.kernel_time(kernel)
     ocip.check(ocip.lib.ocipEnableKernelTiming(b"\x00"))
     return (t1 - t0) * 1e3 / steps, (kms / kn if kn else None)
