@@ -166,7 +166,7 @@ def cpu_baseline(src, tpl, target_s=12.0):
     nthreads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))))
     H, W = src.shape
     th, tw = tpl.shape
-    rows = 4
+    rows = 4 * nthreads   # calibration: a few rows per thread, so the threads are all busy
     t0 = time.perf_counter()
     orc.prox_ref(orc.CC_NORM, src, tpl, rows=(0, rows), nthreads=nthreads)
     per_row = (time.perf_counter() - t0) / rows
