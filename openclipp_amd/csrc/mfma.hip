@@ -31,9 +31,11 @@ __global__ void reset_counters(unsigned *c, int n)
    if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
 }
 
-// 1 / sqrt(SS TT) for every output of a CrossCorr_Norm map, straight from the source image:
-// SS(x, y) = sum of s^2 over the template window of (x, y) (origin (-tw/2, -th/2), zero
-// outside the image), TT = sum t^2 in FP64, the reciprocal square root in FP32 (inv_sqrt).
+// CrossCorr_Norm from the CrossCorr map: every output C(x, y) of the map (written by the MFMA
+// kernel) is scaled in place by 1 / sqrt(SS TT), SS(x, y) = sum of s^2 over the template window
+// of (x, y) (origin (-tw/2, -th/2), zero outside the image) straight from the source image,
+// TT = sum t^2 in FP64, the reciprocal square root in FP32 (inv_sqrt); C * (1 / sqrt(SS TT)) is
+// one FP32 product.
 // One wavefront owns kDenCols output columns and a band of kDenBand output rows:
 //  * vertical pass, lane = column of the window segment [x0 - tw/2, x0 - tw/2 + kDenCols - 1
 //    + tw): two FP64 running sums of s^2 from the band's first row, `lead` taking each row as
@@ -41,13 +43,14 @@ __global__ void reset_counters(unsigned *c, int n)
 //    difference of two prefixes of one sequence (an all-zero column gives exactly 0).  V goes
 //    to LDS as FP32 (relative error 2^-24), kDenRows output rows at a time;
 //  * horizontal pass, lane = (output row, half of the columns): the same pair of running sums
-//    along the row of V gives SS; the results go back through LDS and leave as coalesced rows.
+//    along the row of V gives SS; the results go back through LDS and scale the map's rows
+//    (loaded before the horizontal pass, so that their latency overlaps it).
 // The rows are a stream of steps: step j takes row y0 - th/2 + j into `lead` (j >= 0), row
 // y0 - th/2 + j - th into `lag` (j >= th) and completes output row y0 + j - (th - 1).  Steps
 // run in batches of U, the loads of batch b + 1 in flight while batch b is summed; the
 // stream starts at jstart <= 0 (steps before 0 load nothing) so that output rows fill the
 // LDS block at batch boundaries.  No integral is formed: the source is read from HBM about
-// once (halo columns and the lag rows come from L2) and the map is written once.
+// once (halo columns and the lag rows come from L2) and the map is read and written once.
 constexpr int kDenRows = 32;     // output rows per LDS block
 constexpr int kDenBand = 128;    // output rows per wavefront
 constexpr int kDenCols = 64;     // output columns per wavefront (two halves of 32)
@@ -66,13 +69,14 @@ template <int NCH, bool BUF>
 __global__ __launch_bounds__(64) void ncc_den_src(const char *__restrict__ src, int sstep, int W,
                                                   int H, int tw, int th,
                                                   const double *__restrict__ tsums,
-                                                  float *__restrict__ out, int bands)
+                                                  char *__restrict__ dst, int dstep, int bands)
 {
    extern __shared__ float vbuf[];  // kDenRows x P floats; P odd: conflict-free row walks
    constexpr int P = 64 * NCH + 1;
-   constexpr int U = NCH <= 2 ? 16 : (NCH <= 3 ? 8 : 4);   // steps per batch (divides kDenRows)
+   constexpr int U = NCH <= 3 ? 8 : 4;           // steps per batch (divides kDenRows)
    constexpr int HALF = kDenCols / 2;           // outputs per lane, horizontal pass
-   constexpr int BPB = kDenRows / U;            // output batches per LDS block
+   constexpr int BPB = kDenRows / U;            // output batches per LDS block (even)
+   static_assert(BPB % 2 == 0, "two register sets of batch loads alternate within a block");
    // consecutive blocks (one XCD's run) are side by side along a band: neighbours share the
    // halo columns of the segment in L2
    const int b = xcd_tile(blockIdx.x, gridDim.x);
@@ -94,9 +98,11 @@ __global__ __launch_bounds__(64) void ncc_den_src(const char *__restrict__ src, 
    const __amdgpu_buffer_rsrc_t rsrc =
        __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(src), (short)0, BUF ? H * sstep : 0, 0x00020000);
    const int pre = th - 1;                       // steps before the first output row
-   const int jstart = -((U - pre % U) % U);      // (pre - jstart) % U == 0
+   // jstart <= 0 with (pre - jstart) % U == 0 and an even number b0 of batches before the
+   // first output row (the two register sets of the batch loads alternate with the batch)
+   int jstart = -((U - pre % U) % U);
+   if (((pre - jstart) / U) % 2) jstart -= U;
    const int b0 = (pre - jstart) / U;            // first batch that completes output rows
-   const int nb = b0 + (n + U - 1) / U;
 
    // Every load is unconditional, so a whole batch is in flight at once.  BUF: a buffer load
    // whose offset (row + column, either kDenOob when outside) is out of range returns 0.
@@ -134,7 +140,23 @@ __global__ __launch_bounds__(64) void ncc_den_src(const char *__restrict__ src, 
 #pragma unroll
    for (int k = 0; k < NCH; ++k) lead[k] = lag[k] = 0.0;
 
-   // horizontal pass and stores of the nrows rows of the LDS block (output rows ybase..)
+   // the map's values of output rows [ybase, ybase + nrows): lane = column, unconditional loads
+   // (address select), issued before the block's batches so that waiting for them later does
+   // not wait for the newer batch loads
+   float cval[kDenCols / 64][kDenRows];
+   auto load_cval = [&](int ybase, int nrows) {
+#pragma unroll
+      for (int hh = 0; hh < kDenCols / 64; ++hh) {
+         const int x = x0 + 64 * hh + lane;
+#pragma unroll
+         for (int rr = 0; rr < kDenRows; ++rr) {
+            const float *p = reinterpret_cast<const float *>(dst + (size_t)(ybase + rr) * dstep) + x;
+            cval[hh][rr] = *((x < W && rr < nrows) ? p : &kZeroPad);
+         }
+      }
+   };
+
+   // horizontal pass and scaling of the nrows rows of the LDS block (output rows ybase..)
    auto flush = [&](int ybase, int nrows) {
       __syncthreads();
       const int r = lane & 31, xs = (lane >> 5) * HALF;
@@ -158,15 +180,21 @@ __global__ __launch_bounds__(64) void ncc_den_src(const char *__restrict__ src, 
       }
       __syncthreads();
 #pragma unroll
-      for (int h = 0; h < kDenCols / 64; ++h) {
-         const int x = x0 + 64 * h + lane;
-         if (x < W)
-            for (int rr = 0; rr < nrows; ++rr)
-               out[(size_t)(ybase + rr) * W + x] = vbuf[rr * P + 64 * h + lane];
+      for (int hh = 0; hh < kDenCols / 64; ++hh) {
+         const int x = x0 + 64 * hh + lane;
+         if (x < W) {
+#pragma unroll
+            for (int rr = 0; rr < kDenRows; ++rr)
+               if (rr < nrows)
+                  reinterpret_cast<float *>(dst + (size_t)(ybase + rr) * dstep)[x] =
+                      cval[hh][rr] * vbuf[rr * P + 64 * hh + lane];
+         }
       }
       __syncthreads();
    };
 
+   // vertical pass of batch bb: the running sums, and the column sums V of completed output
+   // rows into the LDS block
    auto process = [&](int bb, const float (&L)[U][NCH], const float (&G)[U][NCH]) {
       const int e0 = (bb - b0) * U;             // output row (band-relative) of step u = 0
 #pragma unroll
@@ -182,37 +210,43 @@ __global__ __launch_bounds__(64) void ncc_den_src(const char *__restrict__ src, 
                vbuf[((e0 + u) % kDenRows) * P + 64 * k + lane] = (float)(lead[k] - lag[k]);
          }
       }
-      if (bb >= b0) {
-         const int done = min(e0 + U, n);        // output rows completed so far
-         if ((bb - b0 + 1) % BPB == 0 || done == n) {
-            const int first = (done - 1) / kDenRows * kDenRows;
-            flush(y0 + first, done - first);
-         }
-      }
    };
 
    float La[U][NCH], Ga[U][NCH], Lb[U][NCH], Gb[U][NCH];
    load_batch(0, La, Ga);
-   for (int bb = 0; bb < nb; bb += 2) {
-      if (bb + 1 < nb) load_batch(bb + 1, Lb, Gb);
+   for (int bb = 0; bb < b0; bb += 2) {
+      load_batch(bb + 1, Lb, Gb);
       process(bb, La, Ga);
-      if (bb + 1 >= nb) break;
-      if (bb + 2 < nb) load_batch(bb + 2, La, Ga);
+      load_batch(bb + 2, La, Ga);
       process(bb + 1, Lb, Gb);
+   }
+   // blocks of kDenRows output rows, BPB (even) batches each
+   for (int first = 0; first < n; first += kDenRows) {
+      const int nrows = min(kDenRows, n - first);
+      load_cval(y0 + first, nrows);
+      const int bbase = b0 + first / U;
+      for (int i = 0; i < BPB; i += 2) {
+         load_batch(bbase + i + 1, Lb, Gb);
+         process(bbase + i, La, Ga);
+         load_batch(bbase + i + 2, La, Ga);
+         process(bbase + i + 1, Lb, Gb);
+      }
+      flush(y0 + first, nrows);
    }
 }
 
 // Wider templates (tw > 64 * kDenMaxChunks - 63): one output per thread, four corners of the
-// FP64 integral each.
-__global__ __launch_bounds__(256) void ncc_inv_den_direct(const double *__restrict__ integ, int W,
-                                                          int H, int tw, int th,
-                                                          const double *__restrict__ tsums,
-                                                          float *__restrict__ out)
+// FP64 integral each, scaling the CrossCorr map in place.
+__global__ __launch_bounds__(256) void ncc_scale_direct(const double *__restrict__ integ, int W,
+                                                        int H, int tw, int th,
+                                                        const double *__restrict__ tsums,
+                                                        char *__restrict__ dst, int dstep)
 {
    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
    if (k >= (long long)W * H) return;
    const int y = (int)(k / W), x = (int)(k - (long long)y * W);
-   out[k] = inv_sqrt(window_sqsum(integ, W, H, x, y, tw, th) * tsums[0]);
+   float *d = reinterpret_cast<float *>(dst + (size_t)y * dstep) + x;
+   *d = *d * inv_sqrt(window_sqsum(integ, W, H, x, y, tw, th) * tsums[0]);
 }
 
 int kc_for(int tw) { return tw <= 97 ? std::max(2, (tw + 31 + 15) / 16) : 6; }
@@ -237,6 +271,99 @@ Geometry geometry(int precision, int KC, int NT, int th_max)
    g.vb = (int)std::min<long>(th_max, rows_fit - (mfma::kTileRows - 1));
    g.lds = g.vb >= 1 ? (size_t)planes_for(precision) * (mfma::kTileRows + g.vb - 1) * g.ldsp * 2 : 0;
    return g;
+}
+
+// Row-pair kernel: chunks of 16 source columns per 16 output columns, and the LDS for
+// 126 + 2 vb staged rows (vb steps of two template rows per band) or for the 129 x
+// (16 NT + 4) FP32 tile of the epilogue, whichever is larger.
+int kc2_for(int tw) { return tw <= 97 ? (tw + 15 + 15) / 16 : (mfma::kBlockW + 15 + 15) / 16; }
+
+Geometry pair_geometry(int KC, int NT, int th)
+{
+   Geometry g;
+   const int lcols = mfma::pair_lcols(KC, NT);
+   g.ldsp = (lcols + 7) / 8 * 8;
+   while ((g.ldsp * 2) % 128 != 16) g.ldsp += 8;
+   const long rows_fit = (long)mfma::kLdsMax / ((long)g.ldsp * 2);
+   const int S = (th + 1) / 2;
+   g.vb = (int)std::min<long>(S, (rows_fit - (mfma::kTileRows - 2)) / 2);
+   const size_t staging = (size_t)(mfma::kTileRows - 2 + 2 * g.vb) * g.ldsp * 2;
+   const size_t epi = (size_t)(mfma::kTileRows + 1) * mfma::pair_tile_pitch(NT) * sizeof(float);
+   g.lds = g.vb >= 1 ? std::max(staging, epi) : 0;
+   return g;
+}
+
+// Strip kernel: a ring of kRingRows LDS rows (pitch = 8 mod 16 elements: the 16-byte A reads of
+// 16 consecutive rows fall on distinct bank groups) and the 4 x 64 FP32 row buffer.
+constexpr int kStripNT = 4;
+
+int ring_ldsp(int lcols) { return (lcols + 7) / 16 * 16 + 8; }
+
+size_t strip_lds(int KC)
+{
+   const int lcols = mfma::pair_lcols(KC, kStripNT);
+   return (size_t)mfma::kRingRows * ring_ldsp(lcols) * 2 + 4 * 16 * kStripNT * sizeof(float);
+}
+
+// rows per step of the ring loads: one 8-column item per thread
+int strip_q_rows(int KC) { return 256 / (mfma::pair_lcols(KC, kStripNT) / 8); }
+
+// The ring schedule of crosscorr_mfma_strip for runs of 1, 2 and 3 tiles (longer runs repeat
+// the steady state of the third): every row read at a step was written to LDS at an earlier
+// step (or staged before the first), and no write replaces a row still to be read.
+bool ring_schedule_ok(int S, int q)
+{
+   const int T = mfma::kPairRows, R = mfma::kRingRows, D = mfma::kRingDepth;
+   for (int K = 1; K <= 3; ++K) {
+      const int grows = T * (K - 1) + 2 * S + mfma::kTileRows - 2;
+      std::vector<int> written(grows, -2), slot(R, -1);
+      const int fill = std::min(grows, mfma::kRingFill);
+      for (int g = 0; g < fill; ++g) {
+         written[g] = -1;
+         slot[g % R] = g;
+      }
+      int ih = fill;
+      std::vector<std::pair<int, int>> batch(K * S, {0, 0});
+      for (int k = 0; k < K; ++k)
+         for (int s = 0; s < S; ++s) {
+            const int n = k * S + s, live = T * k + 2 * s;
+            if (n >= D) {
+               for (int g = batch[n - D].first; g < batch[n - D].second; ++g) {
+                  if (slot[g % R] >= live) return false;
+                  slot[g % R] = g;
+                  written[g] = n;
+               }
+            }
+            const int cnt = std::max(0, std::min(q, std::min(grows, live + R) - ih));
+            batch[n] = {ih, ih + cnt};
+            ih += cnt;
+            for (int g = live; g < live + mfma::kTileRows; ++g)
+               if (g >= grows || written[g] == -2 || written[g] >= n || slot[g % R] != g) return false;
+         }
+      for (int n = std::max(0, K * S - D); n < K * S; ++n)
+         if (batch[n].second > batch[n].first) return false;
+      if (ih != grows) return false;
+   }
+   return true;
+}
+
+bool strip_fits(int precision, int tw, int th)
+{
+   if (precision == mfma::kBF16x3 || tw > 97) return false;
+   const int KC = kc2_for(tw);
+   return KC >= 2 && KC <= 6 && strip_lds(KC) <= (size_t)mfma::kLdsMax &&
+          ring_schedule_ok(mfma::pair_frag_steps(th, 1), strip_q_rows(KC));
+}
+
+int device_cus()
+{
+   static int cus = 0;
+   if (!cus) {
+      int dev = 0;
+      hip_check(hipGetDevice(&dev), "device");
+      hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "device");
+   }
+   return cus;
 }
 
 void launch_single_p(int precision, hipStream_t s, const SingleLaunch &L)
@@ -267,11 +394,15 @@ bool rows_aligned(const View &S)
 }
 
 // ---------------------------------------------------------------------------------------
-// Autotuner: the number of 32-column n-tiles per wave (NT = 2 or 4, i.e. 64- or 128-column
-// output tiles) for each (kind, precision, KC, mode, image size, template height).  The
-// first call with a new key times both variants with HIP events on the call's own inputs
-// and keeps the faster; the table is kept for the process and, when the environment
-// variable OCIP_AUTOTUNE_CACHE names a file, read from and appended to that file.
+// Autotuner: the kernel variant for each (kind, precision, KC, mode, image size, template
+// height).  Variants 2 and 4: crosscorr_mfma with NT = 2 or 4 32-column n-tiles per wave
+// (64- or 128-column output tiles); 14 and 16: crosscorr_mfma_pair with NT = 4 or 6 16-column
+// n-tiles (64- or 96-column tiles); 24: crosscorr_mfma_strip (NT = 4).  The first call with a new key times every candidate
+// with HIP events on the call's own inputs and keeps the fastest; the table is kept for the
+// process and, when the environment variable OCIP_AUTOTUNE_CACHE names a file, read from
+// and appended to that file.
+bool known_variant(int v) { return v == 2 || v == 4 || v == 14 || v == 16 || v == 24; }
+
 std::map<std::string, int> &tune_table()
 {
    static std::map<std::string, int> table;
@@ -283,7 +414,7 @@ std::map<std::string, int> &tune_table()
             char key[256];
             int nt;
             while (fscanf(f, "%255s %d", key, &nt) == 2)
-               if (nt == 2 || nt == 4) table[key] = nt;
+               if (known_variant(nt)) table[key] = nt;
             fclose(f);
          }
       }
@@ -291,7 +422,8 @@ std::map<std::string, int> &tune_table()
    return table;
 }
 
-int tuned_nt(const std::string &key, hipStream_t s, const std::function<void(int)> &run)
+int tuned_variant(const std::string &key, hipStream_t s, const std::function<void(int)> &run,
+                  std::initializer_list<int> candidates)
 {
    auto &table = tune_table();
    auto it = table.find(key);
@@ -300,8 +432,8 @@ int tuned_nt(const std::string &key, hipStream_t s, const std::function<void(int
    hip_check(hipEventCreate(&e0), "event");
    hip_check(hipEventCreate(&e1), "event");
    float best_ms = 1e30f;
-   int best = 4;
-   for (int nt : {4, 2}) {
+   int best = *candidates.begin();
+   for (int nt : candidates) {
       run(nt);  // warm
       hip_check(hipEventRecord(e0, s), "event");
       run(nt);
@@ -336,18 +468,69 @@ std::string tune_key(const char *kind, int precision, int KC, int mode, int W, i
 }  // namespace
 
 // ======================================================================= single template
+// Work buffer: fragments of crosscorr_mfma, then fragments of the row-pair kernels (one plane).
+static size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
 static size_t mfma_frag_bytes(int precision, int tw, int th)
 {
-   const size_t b = (size_t)planes_for(precision) * nub_for(tw) * th * kc_for(tw) * 64 * sizeof(u16x8);
-   return (b + 255) / 256 * 256;
+   return round256((size_t)planes_for(precision) * nub_for(tw) * th * kc_for(tw) * 64 * sizeof(u16x8));
 }
 
-// fragments, then the W x H map of 1 / sqrt(SS TT) of the Norm
+static size_t pair_frag_bytes(int precision, int tw, int th)
+{
+   if (precision == mfma::kBF16x3) return 0;
+   const int nub = nub_for(tw);
+   return round256((size_t)nub * mfma::pair_frag_steps(th, nub) * kc2_for(tw) * 64 * sizeof(u16x8));
+}
+
 bool crosscorr_mfma_needs_integral(int tw) { return den_chunks(tw) > kDenMaxChunks; }
 
-size_t crosscorr_mfma_work_bytes(int precision, int W, int H, int tw, int th)
+size_t crosscorr_mfma_work_bytes(int precision, int, int, int tw, int th)
 {
-   return mfma_frag_bytes(precision, tw, th) + (size_t)W * H * sizeof(float);
+   return mfma_frag_bytes(precision, tw, th) + pair_frag_bytes(precision, tw, th);
+}
+
+// CrossCorr_Norm: the CrossCorr map scaled in place by 1 / sqrt(SS TT)
+static void launch_ncc_scale(hipStream_t s, const View &S, int tw, int th, const double *tsums,
+                             const double *integ, float *dst, int dstep)
+{
+   KernelTimer kt(s, "ncc_den");
+   char *d = reinterpret_cast<char *>(dst);
+   if (den_chunks(tw) <= kDenMaxChunks) {
+      const int bands = (S.height + kDenBand - 1) / kDenBand;
+      const int cblks = (S.width + kDenCols - 1) / kDenCols;
+      const size_t lds = den_lds_bytes(tw);
+      auto go = [&](auto kernel) {
+         if (lds > 64 * 1024)
+            hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                      "lds attribute");
+         kernel<<<bands * cblks, 64, lds, s>>>(static_cast<const char *>(S.ptr), S.step, S.width,
+                                               S.height, tw, th, tsums, d, dstep, bands);
+      };
+      // buffer loads when every byte offset, and two out-of-range offsets added, stay below 2^31
+      const bool buf = (long long)S.height * S.step <= kDenOob;
+#define OCIP_DEN_CASE(N) \
+   case N: if (buf) go(ncc_den_src<N, true>); else go(ncc_den_src<N, false>); break;
+      switch (den_chunks(tw)) {
+      OCIP_DEN_CASE(1)
+      OCIP_DEN_CASE(2)
+      OCIP_DEN_CASE(3)
+      OCIP_DEN_CASE(4)
+      OCIP_DEN_CASE(5)
+      OCIP_DEN_CASE(6)
+      OCIP_DEN_CASE(7)
+      OCIP_DEN_CASE(8)
+      default: if (buf) go(ncc_den_src<9, true>); else go(ncc_den_src<9, false>); break;
+      }
+#undef OCIP_DEN_CASE
+   } else {
+      if (!integ) fail(CL_INVALID_VALUE, "integral required for templates this wide");
+      const long long n = (long long)S.width * S.height;
+      ncc_scale_direct<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(integ, S.width, S.height, tw, th,
+                                                                  tsums, d, dstep);
+   }
+   launch_check("ncc_den");
 }
 
 void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &S, const View &T,
@@ -355,82 +538,108 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
 {
    const int tw = T.width, th = T.height;
    if (tw <= 0 || th <= 0) fail(CL_INVALID_VALUE, "empty template");
-   const int KC = kc_for(tw), nub = nub_for(tw);
+   const int KC = kc_for(tw), KC2 = kc2_for(tw), nub = nub_for(tw);
    if (geometry(precision, KC, 4, th).vb < 1)
       fail(CL_INVALID_VALUE, "template too large for the MFMA tile");
+   const bool pair_ok = precision != mfma::kBF16x3;
+   // the strip kernel's buffer loads and stores address source and output with 32-bit offsets
+   // below kRingOob
+   const bool strip_ok = strip_fits(precision, tw, th) &&
+                         (long long)S.height * S.step <= (long long)mfma::kRingOob &&
+                         (long long)S.height * dstep <= (long long)mfma::kRingOob;
+   // the contraction is the same for CrossCorr and CrossCorr_Norm: one tuning key
+   const std::string key = tune_key("single", precision, KC, mfma::kModeCC, S.width, S.height, th);
+   // OCIP_MFMA_VARIANT (2, 4, 14, 16 or 24) names a variant and skips the autotuner
+   int forced = 0;
+   if (const char *e = getenv("OCIP_MFMA_VARIANT")) {
+      const int v = atoi(e);
+      if (known_variant(v) && (pair_ok || v < 10) && (strip_ok || v != 24)) forced = v;
+   }
+   const auto known = tune_table().find(key);
+   const int chosen = forced ? forced : known == tune_table().end() ? 0 : known->second;
+
+   char *wk = static_cast<char *>(work);
    const size_t nfrag = (size_t)nub * th * KC;
-   u16x8 *f0 = static_cast<u16x8 *>(work);
+   u16x8 *f0 = reinterpret_cast<u16x8 *>(wk);
    u16x8 *f1 = precision == mfma::kBF16x3 ? f0 + nfrag * 64 : nullptr;
+   u16x8 *g0 = reinterpret_cast<u16x8 *>(wk + mfma_frag_bytes(precision, tw, th));
    {
+      // fragments of the chosen variant (of every candidate before the first tuning)
       KernelTimer kt(s, "template_prep");
-      const BatchDesc d{static_cast<const char *>(T.ptr), tw, th, T.step, KC, nub, f0, f1, tsums};
-      if (precision == mfma::kBF16) mfma::launch_single_prep<mfma::kBF16>(s, d);
-      else if (precision == mfma::kF16) mfma::launch_single_prep<mfma::kF16>(s, d);
-      else mfma::launch_single_prep<mfma::kBF16x3>(s, d);
-   }
-   const int mode = norm ? mfma::kModeNCCMap : mfma::kModeCC;
-   float *invden = nullptr;
-   if (norm) {
-      invden = reinterpret_cast<float *>(static_cast<char *>(work) + mfma_frag_bytes(precision, tw, th));
-      KernelTimer kt(s, "ncc_inv_den");
-      const int nch = den_chunks(tw);
-      if (nch <= kDenMaxChunks) {
-         const int bands = (S.height + kDenBand - 1) / kDenBand;
-         const int cblks = (S.width + kDenCols - 1) / kDenCols;
-         const size_t lds = den_lds_bytes(tw);
-         auto go = [&](auto kernel) {
-            if (lds > 64 * 1024)
-               hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
-                         "lds attribute");
-            kernel<<<bands * cblks, 64, lds, s>>>(static_cast<const char *>(S.ptr), S.step,
-                                                  S.width, S.height, tw, th, tsums, invden, bands);
-         };
-         // buffer loads when every byte offset, and two out-of-range offsets added, stay
-         // below 2^31
-         const bool buf = (long long)S.height * S.step <= kDenOob;
-#define OCIP_DEN_CASE(N) \
-   case N: if (buf) go(ncc_den_src<N, true>); else go(ncc_den_src<N, false>); break;
-         switch (nch) {
-         OCIP_DEN_CASE(1)
-         OCIP_DEN_CASE(2)
-         OCIP_DEN_CASE(3)
-         OCIP_DEN_CASE(4)
-         OCIP_DEN_CASE(5)
-         OCIP_DEN_CASE(6)
-         OCIP_DEN_CASE(7)
-         OCIP_DEN_CASE(8)
-         default: if (buf) go(ncc_den_src<9, true>); else go(ncc_den_src<9, false>); break;
-         }
-#undef OCIP_DEN_CASE
-      } else {
-         if (!integ) fail(CL_INVALID_VALUE, "integral required for templates this wide");
-         const long long n = (long long)S.width * S.height;
-         ncc_inv_den_direct<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(integ, S.width, S.height,
-                                                                        tw, th, tsums, invden);
+      if (chosen < 10) {
+         const BatchDesc d{static_cast<const char *>(T.ptr), tw, th, T.step, KC, nub, f0, f1, tsums};
+         if (precision == mfma::kBF16) mfma::launch_single_prep<mfma::kBF16>(s, d);
+         else if (precision == mfma::kF16) mfma::launch_single_prep<mfma::kF16>(s, d);
+         else mfma::launch_single_prep<mfma::kBF16x3>(s, d);
       }
-      launch_check("ncc_inv_den");
+      if (pair_ok && (chosen == 0 || chosen > 10)) {
+         const BatchDesc d{static_cast<const char *>(T.ptr), tw, th, T.step, KC2, nub, g0, nullptr,
+                           tsums};
+         if (precision == mfma::kBF16) mfma::launch_pair_prep<mfma::kBF16>(s, d);
+         else mfma::launch_pair_prep<mfma::kF16>(s, d);
+      }
    }
-   // staged columns start at x0 - tw/2 + 64 ub + 8 g with x0 a multiple of 64: 16-byte
-   // aligned when tw/2 is a multiple of 4 and the rows are 16-byte aligned
+   // staged columns start at x0 - tw/2 + 64 ub + 8 g with x0 a multiple of 64 (crosscorr_mfma)
+   // or 16 (row-pair kernels): 16-byte aligned when tw/2 is a multiple of 4 and the rows are
+   // 16-byte aligned
    const int vec = ((tw / 2) % 4 == 0 && rows_aligned(S)) ? 1 : 0;
-   auto run = [&](int NT) {
-      const Geometry g = geometry(precision, KC, NT, th);
+   auto run = [&](int variant) {
       SingleLaunch L;
+      L.mode = mfma::kModeCC;
+      if (variant == 24) {
+         const int lcols = mfma::pair_lcols(KC2, kStripNT);
+         const int tiles_x = (S.width + 16 * kStripNT - 1) / (16 * kStripNT);
+         const int tiles_y = (S.height + mfma::kPairRows - 1) / mfma::kPairRows;
+         L.a = mfma::TileArgs{static_cast<const float *>(S.ptr), S.width, S.height, S.step, tw,
+                              th, nub, g0, nullptr, tsums, integ, dst, dstep, 0, vec};
+         L.KC = KC2;
+         L.NT = kStripNT;
+         L.ldsp = ring_ldsp(lcols);
+         L.lds = strip_lds(KC2);
+         // two workgroups per CU fit the LDS; OCIP_STRIP_WORKGROUPS sets another count
+         int wgs = 2 * device_cus();
+         if (const char *e = getenv("OCIP_STRIP_WORKGROUPS")) wgs = std::max(1, atoi(e));
+         L.grid = dim3(std::min(tiles_x * tiles_y, wgs), tiles_x);
+         if (precision == mfma::kBF16) mfma::launch_strip<mfma::kBF16>(s, L, tiles_y, strip_q_rows(KC2));
+         else mfma::launch_strip<mfma::kF16>(s, L, tiles_y, strip_q_rows(KC2));
+         return;
+      }
+      if (variant > 10) {
+         const int NT = variant - 10;
+         const Geometry g = pair_geometry(KC2, NT, th);
+         L.a = mfma::TileArgs{static_cast<const float *>(S.ptr), S.width, S.height, S.step, tw,
+                              th, nub, g0, nullptr, tsums, integ, dst, dstep, g.vb, vec};
+         L.KC = KC2;
+         L.NT = NT;
+         L.ldsp = g.ldsp;
+         L.lds = g.lds;
+         L.grid = dim3((S.width + 16 * NT - 1) / (16 * NT),
+                       (S.height + mfma::kPairRows - 1) / mfma::kPairRows);
+         if (precision == mfma::kBF16) mfma::launch_pair<mfma::kBF16>(s, L);
+         else mfma::launch_pair<mfma::kF16>(s, L);
+         return;
+      }
+      const int NT = variant;
+      const Geometry g = geometry(precision, KC, NT, th);
       L.a = mfma::TileArgs{static_cast<const float *>(S.ptr), S.width, S.height, S.step, tw, th,
-                           nub, f0, f1, tsums, integ, invden, dst, dstep, g.vb, vec};
+                           nub, f0, f1, tsums, integ, dst, dstep, g.vb, vec};
       L.KC = KC;
       L.NT = NT;
-      L.mode = mode;
       L.ldsp = g.ldsp;
       L.lds = g.lds;
       L.grid = dim3((S.width + 32 * NT - 1) / (32 * NT),
                     (S.height + mfma::kTileRows - 1) / mfma::kTileRows);
       launch_single_p(precision, s, L);
    };
-   const int NT = tuned_nt(tune_key("single", precision, KC, mode, S.width, S.height, th), s, run);
-   KernelTimer kt(s, "crosscorr_mfma");
-   run(NT);
+   const int variant = forced ? forced
+                       : strip_ok ? tuned_variant(key, s, run, {24, 16, 14, 4, 2})
+                       : pair_ok ? tuned_variant(key, s, run, {16, 14, 4, 2})
+                                 : tuned_variant(key, s, run, {4, 2});
+   {
+      KernelTimer kt(s, "crosscorr_mfma");
+      run(variant);
+   }
+   if (norm) launch_ncc_scale(s, S, tw, th, tsums, integ, dst, dstep);
 }
 
 // ======================================================================= batch
@@ -592,8 +801,8 @@ void run_batch_mfma(hipStream_t s, BatchCache *cache, int num_cus, int precision
                reset();
                bucket_launch(b, NT);
             };
-            nts.push_back(tuned_nt(tune_key("batch", precision, bl[b].KC, mode, S.width, S.height,
-                                            bl[b].th_max), s, run));
+            nts.push_back(tuned_variant(tune_key("batch", precision, bl[b].KC, mode, S.width, S.height,
+                                            bl[b].th_max), s, run, {4, 2}));
          }
       }
       // record the whole batch once

@@ -36,6 +36,36 @@ template <int KC, int NT, int MODE> void batch(hipStream_t s, const BatchLaunch 
                                 L.tiles_x, L.tiles_y, L.counter, L.ldsp);
 }
 
+#if OCIP_MFMA_P != 3
+template <int KC, int NT, int MODE> void pair(hipStream_t s, const SingleLaunch &L)
+{
+   auto k = crosscorr_mfma_pair<P, KC, NT, MODE>;
+   allow_lds(k);
+   const int tiles = L.grid.x * L.grid.y;
+   k<<<tiles, 256, L.lds, s>>>(L.a, L.ldsp, (int)L.grid.x, tiles);
+}
+
+template <int KC, bool VEC> void strip(hipStream_t s, const SingleLaunch &L, int tiles_y, int q_rows)
+{
+   auto k = crosscorr_mfma_strip<P, KC, 4, kModeCC, VEC>;
+   allow_lds(k);
+   k<<<L.grid.x, 256, L.lds, s>>>(L.a, L.ldsp, (int)L.grid.y, tiles_y, q_rows);
+}
+
+// 16-byte ring loads when every 4-column group of the source is wholly inside or outside the
+// image: 16-byte aligned rows (L.a.vec), tw/2 and W multiples of 4
+template <int KC> void strip_mode(hipStream_t s, const SingleLaunch &L, int tiles_y, int q_rows)
+{
+   if (L.a.vec && L.a.W % 4 == 0) strip<KC, true>(s, L, tiles_y, q_rows);
+   else strip<KC, false>(s, L, tiles_y, q_rows);
+}
+
+template <int KC> void pair_mode(hipStream_t s, const SingleLaunch &L)
+{
+   if (L.NT == 4) pair<KC, 4, kModeCC>(s, L); else pair<KC, 6, kModeCC>(s, L);
+}
+#endif
+
 template <int KC, int MODE> void single_nt(hipStream_t s, const SingleLaunch &L)
 {
    if (L.NT == 2) single<KC, 2, MODE>(s, L); else single<KC, 4, MODE>(s, L);
@@ -48,7 +78,7 @@ template <int KC, int MODE> void batch_nt(hipStream_t s, const BatchLaunch &L)
 
 template <int KC> void single_mode(hipStream_t s, const SingleLaunch &L)
 {
-   if (L.mode == kModeCC) single_nt<KC, kModeCC>(s, L); else single_nt<KC, kModeNCCMap>(s, L);
+   single_nt<KC, kModeCC>(s, L);
 }
 
 template <int KC> void batch_mode(hipStream_t s, const BatchLaunch &L)
@@ -87,6 +117,44 @@ template <> void launch_batch<P>(hipStream_t s, const BatchLaunch &L)
    }
    launch_check("crosscorr_mfma_batch");
 }
+
+#if OCIP_MFMA_P != 3
+template <> void launch_pair<P>(hipStream_t s, const SingleLaunch &L)
+{
+   switch (L.KC) {
+   case 1: pair_mode<1>(s, L); break;
+   case 2: pair_mode<2>(s, L); break;
+   case 3: pair_mode<3>(s, L); break;
+   case 4: pair_mode<4>(s, L); break;
+   case 5: pair_mode<5>(s, L); break;
+   case 6: pair_mode<6>(s, L); break;
+   case 7: pair_mode<7>(s, L); break;
+   default: fail(CL_INVALID_VALUE, "template width");
+   }
+   launch_check("crosscorr_mfma_pair");
+}
+
+// grid.x = workgroups, grid.y = tiles_x
+template <> void launch_strip<P>(hipStream_t s, const SingleLaunch &L, int tiles_y, int q_rows)
+{
+   switch (L.KC) {
+   case 2: strip_mode<2>(s, L, tiles_y, q_rows); break;
+   case 3: strip_mode<3>(s, L, tiles_y, q_rows); break;
+   case 4: strip_mode<4>(s, L, tiles_y, q_rows); break;
+   case 5: strip_mode<5>(s, L, tiles_y, q_rows); break;
+   case 6: strip_mode<6>(s, L, tiles_y, q_rows); break;
+   default: fail(CL_INVALID_VALUE, "template width");
+   }
+   launch_check("crosscorr_mfma_strip");
+}
+
+template <> void launch_pair_prep<P>(hipStream_t s, const BatchDesc &d)
+{
+   const int nfrag = d.nub * pair_frag_steps(d.th, d.nub) * d.KC;
+   pair_prep<P><<<(nfrag + 3) / 4, 256, 0, s>>>(d);
+   launch_check("pair_prep");
+}
+#endif
 
 template <> void launch_single_prep<P>(hipStream_t s, const BatchDesc &d)
 {

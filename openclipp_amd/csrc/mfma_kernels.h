@@ -42,8 +42,9 @@ constexpr int kBlockW = 64;             // template column block for wide templa
 constexpr int kLdsMax = 160 * 1024 - 256;
 
 enum { kBF16 = 1, kF16 = 2, kBF16x3 = 3 };
-// kModeNCCMap: CrossCorr_Norm with 1 / sqrt(SS TT) precomputed per output (ncc_inv_den)
-enum { kModeCC = 0, kModeNCC = 1, kModeSSDN = 2, kModeNCCMap = 3 };
+// kModeCC: the CrossCorr map (the single-template call; CrossCorr_Norm scales it in place
+// afterwards); kModeNCC / kModeSSDN: the batch kernel's epilogues from the FP64 integral
+enum { kModeCC = 0, kModeNCC = 1, kModeSSDN = 2 };
 
 // Arguments of one output tile computation.
 struct TileArgs {
@@ -53,7 +54,6 @@ struct TileArgs {
    const u16x8 *frag0, *frag1;   // B fragments (plane 1: lo parts for split BF16)
    const double *tsums;          // [0] = sum t^2
    const double *integ;          // internal exclusive FP64 integral of s^2 (Norm modes)
-   const float *invden;          // 1 / sqrt(SS TT) per output, pitch W (kModeNCCMap)
    float *dst;
    int dstep;
    int vb;                       // template rows per LDS band
@@ -239,6 +239,58 @@ __device__ __forceinline__ void row_mfma(f32x16 (&acc)[NT], const unsigned short
    }
 }
 
+// Stage source rows [row0, row0 + nrows) x columns [col0, col0 + LCOLS) into LDS as 16-bit
+// values (zero outside the image), 8 columns per item, four items per thread in flight.
+template <int P, int LCOLS>
+__device__ __forceinline__ void stage_rows(const TileArgs &a, unsigned short *lds, int ldsp,
+                                           size_t plane, int row0, int col0, int nrows)
+{
+   constexpr int NPL = P == kBF16x3 ? 2 : 1;
+   constexpr int groups = LCOLS / 8;
+   const int W = a.W, H = a.H;
+   const int total = nrows * groups;
+   for (int it0 = threadIdx.x; it0 < total; it0 += 256 * 4) {
+      float f[4][8];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+         const int it = it0 + 256 * k;
+         const int r = it / groups, g = it - r * groups;
+         const int sy = row0 + r, sx = col0 + 8 * g;
+         const bool rowin = it < total && sy >= 0 && sy < H;
+         const float *srow = reinterpret_cast<const float *>(
+             reinterpret_cast<const char *>(a.src) + (size_t)(rowin ? sy : 0) * a.sstep);
+         if (a.vec && rowin && sx >= 0 && sx + 8 <= W) {
+            const float4 q0 = *reinterpret_cast<const float4 *>(srow + sx);
+            const float4 q1 = *reinterpret_cast<const float4 *>(srow + sx + 4);
+            f[k][0] = q0.x; f[k][1] = q0.y; f[k][2] = q0.z; f[k][3] = q0.w;
+            f[k][4] = q1.x; f[k][5] = q1.y; f[k][6] = q1.z; f[k][7] = q1.w;
+         } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+               const int x = sx + i;
+               f[k][i] = (rowin && x >= 0 && x < W) ? srow[x] : 0.f;
+            }
+         }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+         const int it = it0 + 256 * k;
+         if (it >= total) break;
+         const int r = it / groups, g = it - r * groups;
+         u16x8 h0, h1;
+#pragma unroll
+         for (int i = 0; i < 8; ++i) {
+            unsigned short e0, e1;
+            encode<P>(f[k][i], e0, e1);
+            h0[i] = e0;
+            h1[i] = e1;
+         }
+         *reinterpret_cast<u16x8 *>(lds + (size_t)r * ldsp + 8 * g) = h0;
+         if (NPL == 2) *reinterpret_cast<u16x8 *>(lds + plane + (size_t)r * ldsp + 8 * g) = h1;
+      }
+   }
+}
+
 // One 128 x 32NT output tile at tile coordinates (bx, by).  ldsp: LDS row pitch in 16-bit
 // elements.  Starts with a barrier, so consecutive calls may reuse the LDS.
 template <int P, int KC, int NT, int MODE>
@@ -265,50 +317,7 @@ __device__ __forceinline__ void mfma_tile(const TileArgs &a, int bx, int by, uns
          const int nrows = kTileRows + nv - 1;
          const size_t plane = (size_t)nrows * ldsp;
          __syncthreads();
-         // stage rows [y0 - cy + v0, +nrows) x columns [x0 - cx + 64 ub, +lcols), 8 per item,
-         // four items per thread in flight
-         constexpr int groups = lcols / 8;
-         const int total = nrows * groups;
-         for (int it0 = threadIdx.x; it0 < total; it0 += 256 * 4) {
-            float f[4][8];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-               const int it = it0 + 256 * k;
-               const int r = it / groups, g = it - r * groups;
-               const int sy = y0 - cy + v0 + r, sx = x0 - cx + ub * kBlockW + 8 * g;
-               const bool rowin = it < total && sy >= 0 && sy < H;
-               const float *srow = reinterpret_cast<const float *>(
-                   reinterpret_cast<const char *>(a.src) + (size_t)(rowin ? sy : 0) * a.sstep);
-               if (a.vec && rowin && sx >= 0 && sx + 8 <= W) {
-                  const float4 q0 = *reinterpret_cast<const float4 *>(srow + sx);
-                  const float4 q1 = *reinterpret_cast<const float4 *>(srow + sx + 4);
-                  f[k][0] = q0.x; f[k][1] = q0.y; f[k][2] = q0.z; f[k][3] = q0.w;
-                  f[k][4] = q1.x; f[k][5] = q1.y; f[k][6] = q1.z; f[k][7] = q1.w;
-               } else {
-#pragma unroll
-                  for (int i = 0; i < 8; ++i) {
-                     const int x = sx + i;
-                     f[k][i] = (rowin && x >= 0 && x < W) ? srow[x] : 0.f;
-                  }
-               }
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-               const int it = it0 + 256 * k;
-               if (it >= total) break;
-               const int r = it / groups, g = it - r * groups;
-               u16x8 h0, h1;
-#pragma unroll
-               for (int i = 0; i < 8; ++i) {
-                  unsigned short e0, e1;
-                  encode<P>(f[k][i], e0, e1);
-                  h0[i] = e0;
-                  h1[i] = e1;
-               }
-               *reinterpret_cast<u16x8 *>(lds + (size_t)r * ldsp + 8 * g) = h0;
-               if (NPL == 2) *reinterpret_cast<u16x8 *>(lds + plane + (size_t)r * ldsp + 8 * g) = h1;
-            }
-         }
+         stage_rows<P, lcols>(a, lds, ldsp, plane, y0 - cy + v0, x0 - cx + ub * kBlockW, nrows);
          __syncthreads();
 
          // ping-pong: the B fragments of template row v+1 load while row v multiplies
@@ -341,14 +350,7 @@ __device__ __forceinline__ void mfma_tile(const TileArgs &a, int bx, int by, uns
    for (int j = 0; j < NT; ++j) {
       const int x = x0 + 32 * j + n;
       float val[16];
-      if (MODE == kModeNCCMap) {
-         const int xc = x < W ? x : W - 1;
-#pragma unroll
-         for (int i = 0; i < 16; ++i) {
-            const int y = ybase + 8 * (i / 4) + (i % 4);
-            val[i] = acc[j][i] * a.invden[(size_t)(y < H ? y : H - 1) * W + xc];
-         }
-      } else if (MODE != kModeCC) {
+      if (MODE != kModeCC) {
          // four outputs (16 integral corners) at a time keeps the epilogue within the
          // registers of two waves per SIMD
 #pragma unroll
@@ -416,10 +418,498 @@ void crosscorr_mfma_batch(const float *src, int W, int H, int sstep, int vec,
       const int tile = it - (it / tiles) * tiles;
       // 16-byte staging loads: aligned rows (vec) and tw/2 a multiple of 4
       TileArgs a{src, W, H, sstep, b.tw, b.th, b.nub, b.frag0, b.frag1, b.tsums, integ,
-                 nullptr, b.dst, b.dstep, b.vb, (vec && (b.tw / 2) % 4 == 0) ? 1 : 0};
+                 b.dst, b.dstep, b.vb, (vec && (b.tw / 2) % 4 == 0) ? 1 : 0};
       mfma_tile<P, KC, NT, MODE>(a, tile % tiles_x, tile / tiles_x,
                                  reinterpret_cast<unsigned short *>(lds_v), ldsp);
    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Row-pair variant (one 16-bit plane: BF16 or FP16).
+//
+// The 32 MFMA columns n = 16 dy + dx hold 16 output columns dx for two output rows one
+// apart.  Wave w of a workgroup whose outputs are rows [Y0, Y0 + 127) accumulates, in MFMA
+// row m and column half dy, output row y = Y0 - 1 + 32 w + m + dy.  At step s its A rows are
+// source rows Y0 - cy + 32 w + m + 2 s, so one A fragment feeds template row 2 s + 1 to half
+// dy = 0 and template row 2 s to half dy = 1 (rows at or past th are zero):
+//   B_kc[k][16 dy + dx] = t(16 kc + k - dx, 2 s + 1 - dy),
+// with KC = ceil((tw + 15) / 16) chunks of 16 source columns for 16 output columns.  After
+// the last step, half dy = 1 of row m and half dy = 0 of row m + 1 hold the even and the odd
+// template rows of one output row; their sum goes through LDS.  The 4 x 32 rows of each
+// half pair up into 127 complete output rows per workgroup.  Issued MACs per useful MAC:
+// 16 KC / tw x 128 / 127 (1.26 for tw = 64, against 1.5 for the kernel above), with
+// NT + KC - 1 A fragments and KC B fragments per KC NT MFMAs.
+constexpr int kPairRows = kTileRows - 1;   // 127 output rows per workgroup
+
+__host__ __device__ constexpr int pair_lcols(int KC, int NT) { return 16 * (NT + KC - 1); }
+__host__ __device__ constexpr int pair_tile_pitch(int NT) { return 16 * NT + 4; }
+
+// Steps per template column block in the fragment layout of the row-pair kernels: ceil(th / 2),
+// rounded up to even for one-block templates (the strip kernel's step count; the extra step
+// of an odd count has all-zero fragments).
+__host__ __device__ constexpr int pair_frag_steps(int th, int nub)
+{
+   return nub > 1 ? (th + 1) / 2 : ((th + 1) / 2 + 1) / 2 * 2;
+}
+
+// B fragment f = (ub * S + s) * KC + kc, S = pair_frag_steps(th, nub): lane holds B_kc[k][n]
+// for n = lane % 32, k = 8 * (lane / 32) + i.
+template <int P>
+__device__ __forceinline__ void build_pair_fragment(const char *tptr, int tstep, int tw, int th,
+                                                    int KC, int nub, int f, int lane, u16x8 *frag)
+{
+   const int S = pair_frag_steps(th, nub);
+   const int kc = f % KC, s = (f / KC) % S, ub = f / (KC * S);
+   const int n = lane & 31, dy = n >> 4, dx = n & 15, kb = 8 * (lane >> 5);
+   const int v = 2 * s + 1 - dy;
+   const int u0 = nub > 1 ? ub * kBlockW : 0;
+   const int wb = nub > 1 ? min(kBlockW, tw - u0) : tw;
+   const float *trow = reinterpret_cast<const float *>(tptr + (size_t)(v < th ? v : 0) * tstep);
+   u16x8 h;
+#pragma unroll
+   for (int i = 0; i < 8; ++i) {
+      const int u = 16 * kc + kb + i - dx;
+      const float t = (v < th && u >= 0 && u < wb) ? trow[u0 + u] : 0.f;
+      unsigned short h0, h1;
+      encode<P>(t, h0, h1);
+      h[i] = h0;
+   }
+   frag[(size_t)f * 64 + lane] = h;
+}
+
+// Fragments of the row-pair kernel in every block, the FP64 sum of squares in block 0 (same
+// order as single_prep).
+template <int P>
+__global__ __launch_bounds__(256) void pair_prep(BatchDesc d)
+{
+   __shared__ double red[4];
+   const int nfrag = d.nub * pair_frag_steps(d.th, d.nub) * d.KC;
+   const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
+   if (f < nfrag)
+      build_pair_fragment<P>(d.tptr, d.tstep, d.tw, d.th, d.KC, d.nub, f, threadIdx.x & 63,
+                             d.frag0);
+   if (blockIdx.x != 0) return;
+   const int pitch = (d.tw + 15) / 16 * 16;
+   double s2 = 0;
+   for (int k = threadIdx.x; k < pitch * d.th; k += 256) {
+      const int v = k / pitch, u = k - v * pitch;
+      const double t = u < d.tw ? reinterpret_cast<const float *>(d.tptr + (size_t)v * d.tstep)[u] : 0.0;
+      s2 += t * t;
+   }
+   s2 = wave_sum(s2);
+   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s2;
+   __syncthreads();
+   if (threadIdx.x == 0) d.sums[0] = red[0] + red[1] + red[2] + red[3];
+}
+
+// One step: NT + KC - 1 A fragments from the LDS row `ab`, KC B fragments.
+template <int P, int KC, int NT>
+__device__ __forceinline__ void pair_mfma(f32x16 (&acc)[NT], const unsigned short *ab,
+                                          const u16x8 (&b)[KC])
+{
+   constexpr int NA = NT + KC - 1;
+   u16x8 a0[NA];
+#pragma unroll
+   for (int c = 0; c < NA; ++c) a0[c] = *reinterpret_cast<const u16x8 *>(ab + 16 * c);
+#pragma unroll
+   for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) acc[j] = mma<P>(a0[j + kc], b[kc], acc[j]);
+}
+
+// Output tile: 127 rows x 16 NT columns (the CrossCorr map); a.vb = steps (template row pairs)
+// per LDS band.
+template <int P, int KC, int NT, int MODE>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(NT <= 4 ? 3 : 2, NT <= 4 ? 3 : 2)))
+void crosscorr_mfma_pair(TileArgs a, int ldsp, int tiles_x, int tiles)
+{
+   static_assert(P != kBF16x3, "one operand plane");
+   static_assert(MODE == kModeCC, "the CrossCorr map");
+   constexpr int lcols = pair_lcols(KC, NT);
+   constexpr int C = 16 * NT;
+   constexpr int TP = pair_tile_pitch(NT);
+   extern __shared__ u16x8 lds_v[];
+   unsigned short *lds = reinterpret_cast<unsigned short *>(lds_v);
+   const int t = xcd_tile(blockIdx.x, tiles);
+   const int bx = t % tiles_x, by = t / tiles_x;
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   const int x0 = bx * C, Y0 = by * kPairRows;
+   const int cx = a.tw / 2, cy = a.th / 2;
+   const int S = (a.th + 1) / 2;
+
+   f32x16 acc[NT];
+#pragma unroll
+   for (int j = 0; j < NT; ++j)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[j][i] = 0.f;
+
+   for (int ub = 0; ub < a.nub; ++ub) {
+      for (int s0 = 0; s0 < S; s0 += a.vb) {
+         const int ns = min(a.vb, S - s0);
+         __syncthreads();
+         // LDS row r = source row Y0 - cy + 2 s0 + r; wave w reads rows 32 w + m + 2 (s - s0)
+         stage_rows<P, lcols>(a, lds, ldsp, 0, Y0 - cy + 2 * s0, x0 - cx + ub * kBlockW,
+                              kTileRows - 2 + 2 * ns);
+         __syncthreads();
+
+         // ping-pong: the B fragments of step s+1 load while step s multiplies
+         const size_t fb0 = ((size_t)(ub * pair_frag_steps(a.th, a.nub) + s0) * KC) * 64 + lane;
+         u16x8 pa[KC], pb[KC], dummy[KC];
+         load_frags<1, KC>(a.frag0, nullptr, fb0, pa, dummy);
+         const unsigned short *arow = lds + (size_t)(wave * 32 + (lane & 31)) * ldsp + 8 * (lane >> 5);
+         int q = 0;
+         for (; q + 1 < ns; q += 2) {
+            load_frags<1, KC>(a.frag0, nullptr, fb0 + (size_t)(q + 1) * KC * 64, pb, dummy);
+            __builtin_amdgcn_sched_barrier(0);
+            pair_mfma<P, KC, NT>(acc, arow + (size_t)(2 * q) * ldsp, pa);
+            const int qn = q + 2 < ns ? q + 2 : q + 1;
+            load_frags<1, KC>(a.frag0, nullptr, fb0 + (size_t)qn * KC * 64, pa, dummy);
+            __builtin_amdgcn_sched_barrier(0);
+            pair_mfma<P, KC, NT>(acc, arow + (size_t)(2 * q + 2) * ldsp, pb);
+         }
+         if (q < ns) pair_mfma<P, KC, NT>(acc, arow + (size_t)(2 * q) * ldsp, pa);
+      }
+   }
+
+   // Pair the halves in LDS: tile row r = 32 w + m + dy holds output row Y0 - 1 + r.
+   // C fragment: lane holds column n = lane % 32 of each n-tile, rows
+   // m = 8 * (i / 4) + 4 * (lane / 32) + i % 4.
+   float *tile = reinterpret_cast<float *>(lds);
+   const int n = lane & 31, dy = n >> 4, dx = n & 15;
+   const int rb = 32 * wave + 4 * (lane >> 5) + dy;
+   __syncthreads();
+   if (dy) {
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+#pragma unroll
+         for (int i = 0; i < 16; ++i)
+            tile[(rb + 8 * (i / 4) + (i % 4)) * TP + 16 * j + dx] = acc[j][i];
+   }
+   __syncthreads();
+   if (!dy) {
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+#pragma unroll
+         for (int i = 0; i < 16; ++i) {
+            float *p = &tile[(rb + 8 * (i / 4) + (i % 4)) * TP + 16 * j + dx];
+            *p = acc[j][i] + *p;
+         }
+   }
+   __syncthreads();
+   // rows 1..127 are whole; consecutive threads take consecutive columns
+   for (int k = threadIdx.x; k < kPairRows * C; k += 256) {
+      const int r = k / C, c = k - r * C;
+      const int y = Y0 + r, x = x0 + c;
+      if (y >= a.H || x >= a.W) continue;
+      reinterpret_cast<float *>(reinterpret_cast<char *>(a.dst) + (size_t)y * a.dstep)[x] =
+          tile[(r + 1) * TP + c];
+   }
+}
+
+// ---------------------------------------------------------------------------------------
+// Strip variant of the row-pair kernel: persistent workgroups, each taking a contiguous run of
+// 127 x 16 NT tiles down a strip of 16 NT output columns, with the source rows streamed
+// through an LDS ring of kRingRows rows.
+//
+// Ring row g (slot g % kRingRows) holds source row Y00 - cy + g, Y00 the first tile's top
+// output row; tile k of the run reads ring rows 127 k + 32 w + m + 2 s at step s.  Rows
+// [0, kRingFill) are staged before the first step; after that every step (one barrier each)
+// first writes to LDS the rows whose loads were issued kRingDepth steps before, then issues
+// loads for at most q_rows further rows, never past the capacity limit (the first row still
+// read at this step) + kRingRows.  The host accepts (S, q_rows) only when a simulation of this
+// schedule shows that every row read at a step was written at an earlier step.  Every source
+// row of a run is loaded once, and the loads of the next tile's rows overlap this tile's
+// MFMAs.  The two halves of a tile's C fragments are paired with lane shuffles; the top row
+// of waves 1..3 takes the bottom partial row of the wave above through a small LDS buffer,
+// read after the next barrier.
+constexpr int kRingRows = 256;
+constexpr int kRingDepth = 2;
+constexpr int kRingFill = kTileRows + 2 * kRingDepth;   // 132
+
+// ring loads of one step: one 8-column item per thread
+struct RingBatch {
+   float f[8];
+   int lo, cnt;
+};
+
+// Offset that puts a buffer load out of range (the hardware returns 0), for positions outside
+// the image; the strip kernel runs on images of at most kRingOob bytes.
+constexpr unsigned kRingOob = 0x40000000u;
+
+// Byte offsets of the 8 columns of this thread's ring item (column group threadIdx.x % groups,
+// first column col0 + 8 g): VEC, two 16-byte groups, each wholly inside the image or wholly
+// kRingOob (the caller guarantees that no group straddles an image edge); otherwise one per
+// column.
+template <int LCOLS, bool VEC>
+__device__ __forceinline__ void ring_columns(const TileArgs &a, int col0, unsigned (&coff)[VEC ? 2 : 8])
+{
+   constexpr int groups = LCOLS / 8;
+   const int sx = col0 + 8 * (int)(threadIdx.x % groups);
+#pragma unroll
+   for (int i = 0; i < (VEC ? 2 : 8); ++i) {
+      const int x = sx + (VEC ? 4 : 1) * i;
+      coff[i] = (x >= 0 && x < a.W) ? (unsigned)(4 * x) : kRingOob;
+   }
+}
+
+// Ring loads of rows [lo, lo + cnt): unconditional buffer loads of this thread's item, zero
+// outside the image and for items past the batch.
+template <int LCOLS, bool VEC>
+__device__ __forceinline__ void ring_issue(__amdgpu_buffer_rsrc_t rsrc, const TileArgs &a, RingBatch &b,
+                                           int lo, int cnt, int row0,
+                                           const unsigned (&coff)[VEC ? 2 : 8])
+{
+   constexpr int groups = LCOLS / 8;
+   b.lo = lo;
+   b.cnt = cnt;
+   const int r = (int)(threadIdx.x / groups);
+   const int sy = row0 + lo + r;
+   const unsigned roff = (r < cnt && sy >= 0 && sy < a.H) ? (unsigned)(sy * a.sstep) : kRingOob;
+   if constexpr (VEC) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+         const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)(roff + coff[q]), 0, 0);
+#pragma unroll
+         for (int i = 0; i < 4; ++i) b.f[4 * q + i] = __uint_as_float(v[i]);
+      }
+   } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+         b.f[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)(roff + coff[i]), 0, 0));
+   }
+}
+
+template <int P, int LCOLS>
+__device__ __forceinline__ void ring_write(const RingBatch &b, unsigned short *ring, int ldsp)
+{
+   constexpr int groups = LCOLS / 8;
+   const int it = threadIdx.x;
+   const int r = it / groups, g = it - r * groups;
+   if (r >= b.cnt) return;
+   u16x8 h;
+#pragma unroll
+   for (int i = 0; i < 8; ++i) {
+      unsigned short e0, e1;
+      encode<P>(b.f[i], e0, e1);
+      h[i] = e0;
+   }
+   *reinterpret_cast<u16x8 *>(ring + (size_t)((b.lo + r) & (kRingRows - 1)) * ldsp + 8 * g) = h;
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// State of one run of crosscorr_mfma_strip (uniform except `keep`).
+template <int NT>
+struct StripRun {
+   int k, s, ih, grows, row0, col0, x0, Y00;
+   float keep[NT];      // own half (dy = 0) of the top row of the previous tile (lanes 0..15)
+   int keep_y, keep_x0;
+};
+
+// Buffer resources of the strip kernel's source and output: loads at an out-of-range offset
+// (kRingOob) return 0 and stores there are dropped, so no lane branches around them.
+struct StripBufs {
+   __amdgpu_buffer_rsrc_t src, dst;
+};
+
+__device__ __forceinline__ unsigned strip_off(bool ok, int y, int pitch, int x)
+{
+   return ok ? (unsigned)(y * pitch + 4 * x) : kRingOob;
+}
+
+// Value of lane + 16 in rows 0 and 2 (rows of 16 lanes): v_permlane16_swap exchanges the odd
+// rows of its first operand with the even rows of its second.
+__device__ __forceinline__ float lane_plus16(float v)
+{
+   const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+   return __uint_as_float(r[1]);
+}
+
+__device__ __forceinline__ void strip_store(const TileArgs &a, const StripBufs &bufs, bool ok, int y,
+                                            int x, float v)
+{
+   __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), bufs.dst, (int)strip_off(ok, y, a.dstep, x), 0, 0);
+}
+
+template <int MODE, int NT>
+__device__ __forceinline__ void strip_flush(const TileArgs &a, const StripBufs &bufs, StripRun<NT> &st,
+                                            const float *xbuf, int wave, int lane)
+{
+   constexpr int C = 16 * NT;
+   if (st.keep_y < 0) return;
+   const int dx = lane & 15;
+   const bool rowok = wave > 0 && lane < 16 && st.keep_y < a.H;
+#pragma unroll
+   for (int j = 0; j < NT; ++j) {
+      const int x = st.keep_x0 + 16 * j + dx;
+      const float val = st.keep[j] + xbuf[(wave > 0 ? wave - 1 : 0) * C + 16 * j + dx];
+      strip_store(a, bufs, rowok && x < a.W, st.keep_y, x, val);
+   }
+   st.keep_y = -1;
+}
+
+// Epilogue of tile st.k: output row of (m, dy) is Y0 - 1 + 32 wave + m + dy.  Lane
+// L = 16 dy + dx + 32 h holds rows m = 8 (i / 4) + 4 h + i % 4 of column dx in register i; the
+// partner (m - 1, dy = 1) of (m, dy = 0) is register i - 1 of lane L + 16 (i % 4 != 0), register
+// i - 1 of lane L + 48 (i % 4 == 0, h = 0) or register i + 3 of lane L - 16 (i % 4 == 0, h = 1),
+// brought over with permlane swaps.  The bottom partial row of the wave goes to xbuf for the
+// wave below; the wave's own top partial row (i = 0, h = 0) waits in st.keep.
+template <int MODE, int NT>
+__device__ __forceinline__ void strip_epilogue(const TileArgs &a, const StripBufs &bufs,
+                                               StripRun<NT> &st, f32x16 (&acc)[NT], float *xbuf,
+                                               int wave, int lane)
+{
+   constexpr int C = 16 * NT;
+   const int dy = (lane >> 4) & 1, dx = lane & 15, h = lane >> 5;
+   const int Y0 = st.Y00 + kPairRows * st.k;
+   if (lane >= 48) {
+#pragma unroll
+      for (int j = 0; j < NT; ++j) xbuf[wave * C + 16 * j + dx] = acc[j][15];
+   }
+   const int ybase = Y0 - 1 + 32 * wave + 4 * h;
+#pragma unroll
+   for (int j = 0; j < NT; ++j) {
+      __builtin_amdgcn_sched_barrier(0);
+      const int x = st.x0 + 16 * j + dx;
+      const bool colok = dy == 0 && x < a.W;
+      st.keep[j] = acc[j][0];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+         float p;
+         if (i % 4 != 0) {
+            p = lane_plus16(acc[j][i - 1]);
+         } else {
+            // lanes 16..31 of z: register i - 1 of lanes 48..63; lanes 48..63 of z: register
+            // i + 3 of lanes 16..31 (v_permlane32_swap exchanges the upper half of its first
+            // operand with the lower half of its second)
+            const unsigned bv = __float_as_uint(i >= 4 ? acc[j][i - 1] : 0.f);
+            const auto q = __builtin_amdgcn_permlane32_swap(bv, __float_as_uint(acc[j][i + 3]), false, false);
+            const float z = __uint_as_float(h ? q[0] : q[1]);
+            p = lane_plus16(z);
+         }
+         const int y = ybase + 8 * (i / 4) + (i % 4);
+         const bool ok = colok && y < a.H && !(i == 0 && h == 0);
+         strip_store(a, bufs, ok, y, x, acc[j][i] + p);
+      }
+   }
+   st.keep_y = Y0 - 1 + 32 * wave;
+   st.keep_x0 = st.x0;
+}
+
+template <int P, int KC, int NT>
+__device__ __forceinline__ void strip_mfma(f32x16 (&acc)[NT], const unsigned short *ab,
+                                           const u32x4 (&b)[KC])
+{
+   constexpr int NA = NT + KC - 1;
+   u16x8 a0[NA];
+#pragma unroll
+   for (int c = 0; c < NA; ++c) a0[c] = *reinterpret_cast<const u16x8 *>(ab + 16 * c);
+#pragma unroll
+   for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+         acc[j] = mma<P>(a0[j + kc], __builtin_bit_cast(u16x8, b[kc]), acc[j]);
+}
+
+// One step s of tile st.k: barrier, ring write of the batch issued kRingDepth steps ago (same
+// parity), B fragments of the next step, new ring loads, MFMAs.
+template <int P, int KC, int NT, int MODE, bool VEC>
+__device__ __forceinline__ void strip_step(const TileArgs &a, StripRun<NT> &st, f32x16 (&acc)[NT],
+                                           RingBatch &mine, const u32x4 (&bcur)[KC],
+                                           u32x4 (&bnext)[KC], unsigned short *ring, float *xbuf,
+                                           const StripBufs &bufs, const unsigned (&coff)[VEC ? 2 : 8],
+                                           int ldsp, int q_rows, int S, int s, int wave, int lane)
+{
+   constexpr int lcols = pair_lcols(KC, NT);
+   const int live = kPairRows * st.k + 2 * s;
+   __syncthreads();
+   strip_flush<MODE, NT>(a, bufs, st, xbuf, wave, lane);
+   ring_write<P, lcols>(mine, ring, ldsp);
+   const int sn = s + 1 < S ? s + 1 : 0;
+   const u32x4 *fr = reinterpret_cast<const u32x4 *>(a.frag0) + (size_t)sn * KC * 64 + lane;
+#pragma unroll
+   for (int kc = 0; kc < KC; ++kc) bnext[kc] = fr[kc * 64];
+   const int cap = min(st.grows, live + kRingRows);
+   const int cnt = max(0, min(q_rows, cap - st.ih));
+   ring_issue<lcols, VEC>(bufs.src, a, mine, st.ih, cnt, st.row0, coff);
+   st.ih += cnt;
+   __builtin_amdgcn_sched_barrier(0);
+   const int slot = (live + wave * 32 + (lane & 31)) & (kRingRows - 1);
+   strip_mfma<P, KC, NT>(acc, ring + (size_t)slot * ldsp + 8 * (lane >> 5), bcur);
+   __builtin_amdgcn_sched_barrier(0);
+}
+
+// The kernel runs S = pair_frag_steps(th, 1) steps per tile (an even count: for odd
+// ceil(th / 2) the last step's fragments are zero), so that the two register sets of the ring
+// batches and B fragments take the same roles in every tile.
+
+template <int P, int KC, int NT, int MODE, bool VEC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void crosscorr_mfma_strip(TileArgs a, int ldsp, int tiles_x, int tiles_y, int q_rows)
+{
+   static_assert(P != kBF16x3, "one operand plane");
+   static_assert(MODE == kModeCC, "the CrossCorr map");
+   constexpr int lcols = pair_lcols(KC, NT);
+   constexpr int C = 16 * NT;
+   extern __shared__ u16x8 lds_v[];
+   unsigned short *ring = reinterpret_cast<unsigned short *>(lds_v);
+   float *xbuf = reinterpret_cast<float *>(ring + (size_t)kRingRows * ldsp);   // [4][C]
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   const int S = pair_frag_steps(a.th, 1);
+   StripBufs bufs;
+   bufs.src = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(a.src), (short)0, a.H * a.sstep,
+                                                0x00020000);
+   bufs.dst = __builtin_amdgcn_make_buffer_rsrc(a.dst, (short)0, a.H * a.dstep, 0x00020000);
+
+   // this workgroup's run of tiles, strip-major, consecutive runs on one XCD
+   const int total = tiles_x * tiles_y;
+   const int v = xcd_tile(blockIdx.x, gridDim.x);
+   const int t_end = (int)((long long)(v + 1) * total / gridDim.x);
+   StripRun<NT> st;
+   st.keep_y = -1;
+   for (int t = (int)((long long)v * total / gridDim.x); t < t_end;) {
+      const int bx = t / tiles_y, by0 = t - bx * tiles_y;
+      const int K = min(tiles_y - by0, t_end - t);
+      t += K;
+      st.x0 = bx * C;
+      st.Y00 = by0 * kPairRows;
+      st.row0 = st.Y00 - a.th / 2;
+      st.col0 = st.x0 - a.tw / 2;
+      st.grows = kPairRows * (K - 1) + 2 * S + kTileRows - 2;
+
+      __syncthreads();   // the previous run's reads of the ring and xbuf are done
+      strip_flush<MODE, NT>(a, bufs, st, xbuf, wave, lane);
+      const int fill = min(st.grows, kRingFill);
+      stage_rows<P, lcols>(a, ring, ldsp, 0, st.row0, st.col0, fill);
+      unsigned coff[VEC ? 2 : 8];
+      ring_columns<lcols, VEC>(a, st.col0, coff);
+      st.ih = fill;
+      RingBatch b0, b1;
+      b0.cnt = 0;
+      b1.cnt = 0;
+      u32x4 f0[KC], f1[KC];
+      const u32x4 *fr = reinterpret_cast<const u32x4 *>(a.frag0) + lane;
+#pragma unroll
+      for (int kc = 0; kc < KC; ++kc) f0[kc] = fr[kc * 64];
+      __syncthreads();   // the cold rows are in LDS
+      for (st.k = 0; st.k < K; ++st.k) {
+         f32x16 acc[NT];
+#pragma unroll
+         for (int j = 0; j < NT; ++j)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[j][i] = 0.f;
+         for (int s = 0; s < S; s += 2) {
+            strip_step<P, KC, NT, MODE, VEC>(a, st, acc, b0, f0, f1, ring, xbuf, bufs, coff, ldsp,
+                                             q_rows, S, s, wave, lane);
+            strip_step<P, KC, NT, MODE, VEC>(a, st, acc, b1, f1, f0, ring, xbuf, bufs, coff, ldsp,
+                                             q_rows, S, s + 1, wave, lane);
+         }
+         strip_epilogue<MODE, NT>(a, bufs, st, acc, xbuf, wave, lane);
+      }
+   }
+   __syncthreads();
+   strip_flush<MODE, NT>(a, bufs, st, xbuf, wave, lane);
 }
 
 #endif  // __HIPCC__
@@ -449,6 +939,9 @@ struct BatchLaunch {
 template <int P> void launch_single(hipStream_t s, const SingleLaunch &L);
 template <int P> void launch_batch(hipStream_t s, const BatchLaunch &L);
 template <int P> void launch_single_prep(hipStream_t s, const BatchDesc &d);
+template <int P> void launch_pair(hipStream_t s, const SingleLaunch &L);
+template <int P> void launch_pair_prep(hipStream_t s, const BatchDesc &d);
+template <int P> void launch_strip(hipStream_t s, const SingleLaunch &L, int tiles_y, int q_rows);
 template <int P> void launch_batch_prep(hipStream_t s, const BatchDesc *descs, int n);
 
 }  // namespace mfma

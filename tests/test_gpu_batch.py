@@ -43,6 +43,13 @@ def run_batch(ocip, fn, src, tpls, prec, S=None, Ts=None, Ds=None):
     return [d.read().copy() for d in Ds], (S, Ts, Ds)
 
 
+@pytest.fixture(autouse=True)
+def single_calls_use_the_batch_tile_code(monkeypatch):
+    """The single-template calls compared here run crosscorr_mfma (variant 4), the tile code
+    of the batch kernel, so that the per-pixel accumulation order is the same."""
+    monkeypatch.setenv("OCIP_MFMA_VARIANT", "4")
+
+
 def same_map(batch, single, exact):
     """Batch map against the single-template call: identical NaN positions and values equal
     (exact) or within 1e-6 relative (the denominators' rounding)."""

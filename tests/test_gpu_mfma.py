@@ -133,6 +133,67 @@ def test_wide_template_reads_the_integral(ocip):
     assert np.all(np.abs(got - ncc64) <= tol * (1 + np.abs(ncc64)))
 
 
+# crosscorr_mfma NT = 2, 4; crosscorr_mfma_pair NT = 4, 6; crosscorr_mfma_strip (where its ring
+# schedule fits the template; elsewhere the autotuner's choice)
+VARIANTS = [2, 4, 14, 16, 24]
+VARIANT_CASES = [((150, 260), (1, 1)), ((150, 260), (64, 64)), ((300, 200), (31, 31)),
+                 ((131, 97), (97, 5)), ((260, 140), (3, 130)), ((129, 129), (16, 17)),
+                 ((40, 620), (530, 3)), ((7, 300), (12, 7))]
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("prec", ["bf16", "f16"])
+def test_kernel_variants(ocip, monkeypatch, prec, variant):
+    """Every tile variant the autotuner may pick, forced through OCIP_MFMA_VARIANT: CC within
+    the operand-precision bound of the FP64 oracle, NCC within the bound over the exact
+    denominator, and an exact crop giving NCC = 1 at its centre."""
+    monkeypatch.setenv("OCIP_MFMA_VARIANT", str(variant))
+    for k, ((H, W), (tw, th)) in enumerate(VARIANT_CASES):
+        rng = np.random.default_rng(100 + k)
+        src = rng.random((H, W), dtype=np.float32) - np.float32(0.25)
+        tpl = rng.random((th, tw), dtype=np.float32) - np.float32(0.25)
+        say("variant", variant, prec, (H, W), (tw, th))
+        f64 = orc.prox_f64(orc.CC, src, tpl)
+        b = bound(prec, src, tpl)
+        got = run_op(ocip, "CrossCorr", src, tpl, precision=PREC[prec]).astype(np.float64)
+        assert np.all(np.abs(got - f64) <= b), ((H, W), (tw, th), float(np.max(np.abs(got - f64) / b)))
+        ncc64 = orc.prox_f64(orc.CC_NORM, src, tpl)
+        s64, t64 = src.astype(np.float64), tpl.astype(np.float64)
+        den = np.sqrt(orc.prox_f64(orc.CC, s64 * s64, np.ones_like(t64)) * (t64 * t64).sum())
+        got = run_op(ocip, "CrossCorr_Norm", src, tpl, precision=PREC[prec]).astype(np.float64)
+        ok = np.isfinite(ncc64)
+        cond = 16 * 2.0 ** -53 * integral_kappa(src, tpl) * np.abs(ncc64) + 2.0 ** -22
+        assert np.all(np.abs(got - ncc64)[ok] <= (b / den + cond)[ok] * 1.01 + 1e-7), ((H, W), (tw, th))
+    # exact crop: NCC = 1 (to the operand rounding) at the crop's centre
+    rng = np.random.default_rng(7)
+    src = rng.random((300, 330), dtype=np.float32) + np.float32(0.1)
+    tpl = np.ascontiguousarray(src[100:164, 150:214])
+    got = run_op(ocip, "CrossCorr_Norm", src, tpl, precision=PREC[prec])
+    assert np.unravel_index(np.nanargmax(got), got.shape) == (132, 182)
+    assert abs(float(got[132, 182]) - 1.0) <= 2 * UNIT[prec]
+
+
+@pytest.mark.parametrize("wgs", [1, 3, 7, 0])
+def test_strip_runs(ocip, monkeypatch, wgs):
+    """crosscorr_mfma_strip with few workgroups, so that each takes long runs of tiles down its
+    strips and across strip boundaries (0: the default count); CC and NCC against FP64."""
+    monkeypatch.setenv("OCIP_MFMA_VARIANT", "24")
+    if wgs:
+        monkeypatch.setenv("OCIP_STRIP_WORKGROUPS", str(wgs))
+    rng = np.random.default_rng(40 + wgs)
+    for (H, W), (tw, th) in [((700, 150), (20, 24)), ((520, 64), (64, 64)), ((389, 200), (33, 17))]:
+        src = rng.random((H, W), dtype=np.float32) - np.float32(0.2)
+        tpl = rng.random((th, tw), dtype=np.float32) - np.float32(0.2)
+        say("strip", wgs, (H, W), (tw, th))
+        f64 = orc.prox_f64(orc.CC, src, tpl)
+        got = run_op(ocip, "CrossCorr", src, tpl, precision=1).astype(np.float64)
+        assert np.all(np.abs(got - f64) <= bound("bf16", src, tpl)), ((H, W), (tw, th))
+        ncc64 = orc.prox_f64(orc.CC_NORM, src, tpl)
+        got = run_op(ocip, "CrossCorr_Norm", src, tpl, precision=1).astype(np.float64)
+        ok = np.isfinite(ncc64)
+        assert np.max(np.abs(got - ncc64)[ok]) <= 2 * UNIT["bf16"] + 1e-6
+
+
 def test_f32_precision_is_the_plain_path(ocip):
     rng = np.random.default_rng(5)
     src = rng.random((70, 90), dtype=np.float32)
