@@ -66,14 +66,14 @@ size_t den_lds_bytes(int tw) { return (size_t)kDenRows * (den_chunks(tw) * 64 + 
 constexpr int kDenOob = 0x40000000;
 
 template <int NCH, bool BUF>
-__global__ __launch_bounds__(64) void ncc_den_src(const char *__restrict__ src, int sstep, int W,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NCH <= 3 ? 2 : 1, NCH <= 6 ? 2 : 1))) void ncc_den_src(const char *__restrict__ src, int sstep, int W,
                                                   int H, int tw, int th,
                                                   const double *__restrict__ tsums,
                                                   char *__restrict__ dst, int dstep, int bands)
 {
    extern __shared__ float vbuf[];  // kDenRows x P floats; P odd: conflict-free row walks
    constexpr int P = 64 * NCH + 1;
-   constexpr int U = NCH <= 3 ? 8 : 4;           // steps per batch (divides kDenRows)
+   constexpr int U = 4;                          // steps per batch (divides kDenRows)
    constexpr int HALF = kDenCols / 2;           // outputs per lane, horizontal pass
    constexpr int BPB = kDenRows / U;            // output batches per LDS block (even)
    static_assert(BPB % 2 == 0, "two register sets of batch loads alternate within a block");
@@ -98,10 +98,13 @@ __global__ __launch_bounds__(64) void ncc_den_src(const char *__restrict__ src, 
    const __amdgpu_buffer_rsrc_t rsrc =
        __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(src), (short)0, BUF ? H * sstep : 0, 0x00020000);
    const int pre = th - 1;                       // steps before the first output row
-   // jstart <= 0 with (pre - jstart) % U == 0 and an even number b0 of batches before the
-   // first output row (the two register sets of the batch loads alternate with the batch)
+   // D register sets of batch loads rotate with the batch (D - 1 batches in flight ahead of
+   // the one being summed); jstart <= 0 with (pre - jstart) % U == 0 and a multiple b0 of D
+   // batches before the first output row, so that every block starts on set 0
+   constexpr int D = NCH <= 3 ? 4 : 2;
+   static_assert(BPB % D == 0, "blocks hold whole rotations of the batch registers");
    int jstart = -((U - pre % U) % U);
-   if (((pre - jstart) / U) % 2) jstart -= U;
+   while (((pre - jstart) / U) % D) jstart -= U;
    const int b0 = (pre - jstart) / U;            // first batch that completes output rows
 
    // Every load is unconditional, so a whole batch is in flight at once.  BUF: a buffer load
@@ -212,25 +215,25 @@ __global__ __launch_bounds__(64) void ncc_den_src(const char *__restrict__ src, 
       }
    };
 
-   float La[U][NCH], Ga[U][NCH], Lb[U][NCH], Gb[U][NCH];
-   load_batch(0, La, Ga);
-   for (int bb = 0; bb < b0; bb += 2) {
-      load_batch(bb + 1, Lb, Gb);
-      process(bb, La, Ga);
-      load_batch(bb + 2, La, Ga);
-      process(bb + 1, Lb, Gb);
-   }
+   float Ls[D][U][NCH], Gs[D][U][NCH];
+#pragma unroll
+   for (int d = 0; d < D - 1; ++d) load_batch(d, Ls[d], Gs[d]);
+   // batches bb .. bb + D - 1 (bb a multiple of D): each step loads batch + D - 1 into the set
+   // freed by the previous batch, then sums its own
+   auto rotation = [&](int bb) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+         load_batch(bb + d + D - 1, Ls[(d + D - 1) % D], Gs[(d + D - 1) % D]);
+         process(bb + d, Ls[d], Gs[d]);
+      }
+   };
+   for (int bb = 0; bb < b0; bb += D) rotation(bb);
    // blocks of kDenRows output rows, BPB (even) batches each
    for (int first = 0; first < n; first += kDenRows) {
       const int nrows = min(kDenRows, n - first);
       load_cval(y0 + first, nrows);
       const int bbase = b0 + first / U;
-      for (int i = 0; i < BPB; i += 2) {
-         load_batch(bbase + i + 1, Lb, Gb);
-         process(bbase + i, La, Ga);
-         load_batch(bbase + i + 2, La, Ga);
-         process(bbase + i + 1, Lb, Gb);
-      }
+      for (int i = 0; i < BPB; i += D) rotation(bbase + i);
       flush(y0 + first, nrows);
    }
 }
