@@ -133,6 +133,32 @@ __device__ __forceinline__ void build_fragment(const char *tptr, int tstep, int 
    if (P == kBF16x3) frag1[(size_t)f * 64 + lane] = b;
 }
 
+// FP64 sum of t^2 over the template into d.sums[0], by one 256-thread block: rows padded to 16
+// columns, thread k takes elements k, k + 256, ... in that order (eight loads in flight), then
+// the wave sums and the four waves in order.  Every prep kernel uses it, so batch and single
+// calls give identical sums.
+__device__ __forceinline__ void template_sqsum(const BatchDesc &d, double *red)
+{
+   const int pitch = (d.tw + 15) / 16 * 16;
+   const int total = pitch * d.th;
+   double s2 = 0;
+   for (int k0 = threadIdx.x; k0 < total; k0 += 256 * 8) {
+      float t[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+         const int k = k0 + 256 * i;
+         const int v = k / pitch, u = k - v * pitch;
+         t[i] = (k < total && u < d.tw) ? reinterpret_cast<const float *>(d.tptr + (size_t)v * d.tstep)[u] : 0.f;
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) s2 += (double)t[i] * (double)t[i];
+   }
+   s2 = wave_sum(s2);
+   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s2;
+   __syncthreads();
+   if (threadIdx.x == 0) d.sums[0] = red[0] + red[1] + red[2] + red[3];
+}
+
 // Batch: one workgroup per template builds its fragments and its FP64 sum of squares.
 template <int P>
 __global__ __launch_bounds__(256) void batch_prep(const BatchDesc *__restrict__ descs)
@@ -143,19 +169,7 @@ __global__ __launch_bounds__(256) void batch_prep(const BatchDesc *__restrict__ 
    for (int k = threadIdx.x; k < nfrag * 64; k += 256)
       build_fragment<P>(d.tptr, d.tstep, d.tw, d.th, d.KC, d.nub, k >> 6, k & 63, d.frag0,
                         d.frag1);
-   // same summation order as the single-template staging (rows padded to 16 columns,
-   // 256 threads striding), so batch and single calls give identical sums
-   const int pitch = (d.tw + 15) / 16 * 16;
-   double s2 = 0;
-   for (int k = threadIdx.x; k < pitch * d.th; k += 256) {
-      const int v = k / pitch, u = k - v * pitch;
-      const double t = u < d.tw ? reinterpret_cast<const float *>(d.tptr + (size_t)v * d.tstep)[u] : 0.0;
-      s2 += t * t;
-   }
-   s2 = wave_sum(s2);
-   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s2;
-   __syncthreads();
-   if (threadIdx.x == 0) d.sums[0] = red[0] + red[1] + red[2] + red[3];
+   template_sqsum(d, red);
 }
 
 // Single template straight from its image: fragments in every block, the FP64 sum of
@@ -170,17 +184,7 @@ __global__ __launch_bounds__(256) void single_prep(BatchDesc d)
       build_fragment<P>(d.tptr, d.tstep, d.tw, d.th, d.KC, d.nub, f, threadIdx.x & 63, d.frag0,
                         d.frag1);
    if (blockIdx.x != 0) return;
-   const int pitch = (d.tw + 15) / 16 * 16;
-   double s2 = 0;
-   for (int k = threadIdx.x; k < pitch * d.th; k += 256) {
-      const int v = k / pitch, u = k - v * pitch;
-      const double t = u < d.tw ? reinterpret_cast<const float *>(d.tptr + (size_t)v * d.tstep)[u] : 0.0;
-      s2 += t * t;
-   }
-   s2 = wave_sum(s2);
-   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s2;
-   __syncthreads();
-   if (threadIdx.x == 0) d.sums[0] = red[0] + red[1] + red[2] + red[3];
+   template_sqsum(d, red);
 }
 
 template <int P>
@@ -489,17 +493,7 @@ __global__ __launch_bounds__(256) void pair_prep(BatchDesc d)
       build_pair_fragment<P>(d.tptr, d.tstep, d.tw, d.th, d.KC, d.nub, f, threadIdx.x & 63,
                              d.frag0);
    if (blockIdx.x != 0) return;
-   const int pitch = (d.tw + 15) / 16 * 16;
-   double s2 = 0;
-   for (int k = threadIdx.x; k < pitch * d.th; k += 256) {
-      const int v = k / pitch, u = k - v * pitch;
-      const double t = u < d.tw ? reinterpret_cast<const float *>(d.tptr + (size_t)v * d.tstep)[u] : 0.0;
-      s2 += t * t;
-   }
-   s2 = wave_sum(s2);
-   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s2;
-   __syncthreads();
-   if (threadIdx.x == 0) d.sums[0] = red[0] + red[1] + red[2] + red[3];
+   template_sqsum(d, red);
 }
 
 // One step: NT + KC - 1 A fragments from the LDS row `ab`, KC B fragments.
