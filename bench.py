@@ -134,6 +134,78 @@ def fft_bench(ocip, src, steps):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "crosscorr_norm_4096_t64_traffic.md")
 
 
+def configs_bench(ocip, steps):
+    """The other template-matching workloads, each timed like the headline (wall clock over
+    `steps` calls between two ocipFinish, kernel averages from HIP events in a second run):
+    the 1024^2 F32 / 16x16 plain path (SqrDistance and CrossCorr_Norm; for a template this
+    small the kernel is bounded by HBM, so its achieved GB/s are given with the algorithmic
+    bytes source + map + template), the 4096^2 U8 / 32x32 integer workload (all five ops,
+    exact integer sums), and the gradients of the 16 x 1024^2 / 31x31 workload (both backward
+    calls per image, 16 images per step)."""
+    from openclipp_amd import workloads
+    out = {}
+
+    src, tpl, _ = workloads.plain_1024()
+    H, W = src.shape
+    th, tw = tpl.shape
+    S, T = ocip.Image(src).send(), ocip.Image(tpl).send()
+    D = ocip.Image(shape=(H, W), dtype=np.float32)
+    plain = {"workload": f"{W}x{H} F32 U[0,1), {tw}x{th} crop + N(0, 0.01^2), plain F32 path"}
+    for op in ("SqrDistance", "CrossCorr_Norm"):
+        ms, kms = time_steps(ocip, lambda: ocip.proximity(op, S, T, D), steps, 2, "prox_tile_f32")
+        alg = W * H * 4 * 2 + tw * th * 4
+        plain[op] = {"ms_per_step": round(ms, 4), "kernel_ms": round(kms, 4) if kms else None,
+                     "gpix_tpix_per_s": round(W * H * tw * th / (ms * 1e-3) / 1e9, 1),
+                     "kernel_hbm_gbps": round(alg / (kms * 1e-3) / 1e9, 1) if kms else None,
+                     "hbm_peak_gbps": 8000.0}
+    out["plain_1024"] = plain
+    for i in (S, T, D):
+        i.release()
+
+    src, tpl, _ = workloads.integer_4096()
+    H, W = src.shape
+    th, tw = tpl.shape
+    S, T = ocip.Image(src).send(), ocip.Image(tpl).send()
+    D = ocip.Image(shape=(H, W), dtype=np.float32)
+    integer = {"workload": f"{W}x{H} U8 blurred field, {tw}x{th} crop + U{{-3..3}}, exact integer sums"}
+    for op in ocip.PROXIMITY_OPS:
+        ms, kms = time_steps(ocip, lambda: ocip.proximity(op, S, T, D), max(3, steps // 2), 1,
+                             "prox_tile_int")
+        integer[op] = {"ms_per_step": round(ms, 4), "kernel_ms": round(kms, 4) if kms else None,
+                       "gpix_tpix_per_s": round(W * H * tw * th / (ms * 1e-3) / 1e9, 1)}
+    out["integer_4096"] = integer
+    for i in (S, T, D):
+        i.release()
+
+    srcs, tpl, w_ncc, w_ssd = workloads.diff_1024x16()
+    n, H, W = srcs.shape
+    th, tw = tpl.shape
+    T = ocip.Image(tpl).send()
+    Ss = [ocip.Image(srcs[k]).send() for k in range(n)]
+    G1 = [ocip.Image(w_ncc[k]).send() for k in range(n)]
+    G2 = [ocip.Image(w_ssd[k]).send() for k in range(n)]
+    dS = ocip.Image(shape=(H, W), dtype=np.float32)
+    dT = ocip.Image(shape=(th, tw), dtype=np.float32)
+
+    def grads():
+        for k in range(n):
+            ocip.check(ocip.lib.ocipCrossCorr_NormBackward(Ss[k].handle, T.handle, G1[k].handle,
+                                                            dS.handle, dT.handle))
+            ocip.check(ocip.lib.ocipSqrDistanceBackward(Ss[k].handle, T.handle, G2[k].handle,
+                                                         dS.handle, dT.handle))
+        return ocip.CL_SUCCESS
+
+    ms, kms = time_steps(ocip, grads, max(2, steps // 8), 1, "backward")
+    out["diff_1024x16"] = {
+        "workload": f"{n} x {W}x{H} F32 U[0,1), {tw}x{th} N(0,1) template: CrossCorr_Norm and "
+                    f"SqrDistance gradients wrt images and template, {n} images per step",
+        "ms_per_step": round(ms, 4),
+        "gradient_gpix_tpix_per_s": round(2 * 2 * n * W * H * tw * th / (ms * 1e-3) / 1e9, 1)}
+    for i in Ss + G1 + G2 + [T, dS, dT]:
+        i.release()
+    return out
+
+
 def measured_traffic():
     """HBM bytes per launch of the headline kernel (CrossCorr_Norm, BF16, this workload) from
     the committed summary of two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE), each scaled
@@ -190,6 +262,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-fft", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the plain 1024^2, integer 4096^2 and gradient workloads")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB device buffer (scale for rocprofv3 FETCH_SIZE/"
                          "WRITE_SIZE readings)")
@@ -270,6 +344,8 @@ def main():
         line["batch_2048"] = batch_bench(ocip, max(2, args.steps // 4))
     if not args.no_fft:
         line["fft_4096"] = fft_bench(ocip, src, max(2, args.steps // 4))
+    if not args.no_configs:
+        line["configs"] = configs_bench(ocip, max(4, args.steps // 2))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(src, tpl)
     for i in (S, T, D, D32):
