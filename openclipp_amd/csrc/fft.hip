@@ -196,13 +196,10 @@ __device__ __forceinline__ void lds_fft(double2 *x, int nseq, const FftPlan *__r
    }
 }
 
-constexpr int kRowRegs = (kFftMaxLen + kFftThreads - 1) / kFftThreads;   // per thread, one row
 
-// The three passes are persistent: gridDim.x workgroups (one per CU, a multiple of 8) take
-// items t * gridDim.x + slot, slot = xcd_tile(blockIdx.x): at any time the workgroups of one
-// XCD work on consecutive items, so the transposed 16-byte stores of neighbouring items fill
-// the same lines of that XCD's L2.  The next item's input is loaded into registers while the
-// current one is transformed in LDS.
+// One workgroup per row (passes 1 and 3) or column pair (pass 2).  Passes 1 and 2 take their
+// rows in XCD-aware order (xcd_tile): the workgroups of one XCD take consecutive rows, so the
+// transposed 16-byte stores of neighbouring rows fill the same lines of that XCD's L2.
 
 // Pass 1: row y of z = s + i t (s at (tw/2, th/2), t at (0, 0), zero elsewhere), FFT along
 // x, stored transposed: out[kx * Hb + y].

@@ -726,7 +726,7 @@ __device__ __forceinline__ void strip_store(const TileArgs &a, const StripBufs &
    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), bufs.dst, (int)strip_off(ok, y, a.dstep, x), 0, 0);
 }
 
-template <int MODE, int NT>
+template <int NT>
 __device__ __forceinline__ void strip_flush(const TileArgs &a, const StripBufs &bufs, StripRun<NT> &st,
                                             const float *xbuf, int wave, int lane)
 {
@@ -749,7 +749,7 @@ __device__ __forceinline__ void strip_flush(const TileArgs &a, const StripBufs &
 // i - 1 of lane L + 48 (i % 4 == 0, h = 0) or register i + 3 of lane L - 16 (i % 4 == 0, h = 1),
 // brought over with permlane swaps.  The bottom partial row of the wave goes to xbuf for the
 // wave below; the wave's own top partial row (i = 0, h = 0) waits in st.keep.
-template <int MODE, int NT>
+template <int NT>
 __device__ __forceinline__ void strip_epilogue(const TileArgs &a, const StripBufs &bufs,
                                                StripRun<NT> &st, f32x16 (&acc)[NT], float *xbuf,
                                                int wave, int lane)
@@ -818,7 +818,7 @@ __device__ __forceinline__ void strip_step(const TileArgs &a, StripRun<NT> &st, 
    constexpr int lcols = pair_lcols(KC, NT);
    const int live = kPairRows * st.k + 2 * s;
    __syncthreads();
-   strip_flush<MODE, NT>(a, bufs, st, xbuf, wave, lane);
+   strip_flush<NT>(a, bufs, st, xbuf, wave, lane);
    ring_write<P, lcols>(mine, ring, ldsp);
    const int sn = s + 1 < S ? s + 1 : 0;
    const u32x4 *fr = reinterpret_cast<const u32x4 *>(a.frag0) + (size_t)sn * KC * 64 + lane;
@@ -873,7 +873,7 @@ void crosscorr_mfma_strip(TileArgs a, int ldsp, int tiles_x, int tiles_y, int q_
       st.grows = kPairRows * (K - 1) + 2 * S + kTileRows - 2;
 
       __syncthreads();   // the previous run's reads of the ring and xbuf are done
-      strip_flush<MODE, NT>(a, bufs, st, xbuf, wave, lane);
+      strip_flush<NT>(a, bufs, st, xbuf, wave, lane);
       const int fill = min(st.grows, kRingFill);
       stage_rows<P, lcols>(a, ring, ldsp, 0, st.row0, st.col0, fill);
       unsigned coff[VEC ? 2 : 8];
@@ -899,11 +899,11 @@ void crosscorr_mfma_strip(TileArgs a, int ldsp, int tiles_x, int tiles_y, int q_
             strip_step<P, KC, NT, MODE, VEC>(a, st, acc, b1, f1, f0, ring, xbuf, bufs, coff, ldsp,
                                              q_rows, S, s + 1, wave, lane);
          }
-         strip_epilogue<MODE, NT>(a, bufs, st, acc, xbuf, wave, lane);
+         strip_epilogue<NT>(a, bufs, st, acc, xbuf, wave, lane);
       }
    }
    __syncthreads();
-   strip_flush<MODE, NT>(a, bufs, st, xbuf, wave, lane);
+   strip_flush<NT>(a, bufs, st, xbuf, wave, lane);
 }
 
 #endif  // __HIPCC__
