@@ -173,6 +173,26 @@ def test_kernel_variants(ocip, monkeypatch, prec, variant):
     assert abs(float(got[132, 182]) - 1.0) <= 2 * UNIT[prec]
 
 
+@pytest.mark.parametrize("prec", ["bf16", "f16"])
+def test_row_pair_variants_are_bit_identical(ocip, monkeypatch, prec):
+    """The row-pair variants (14, 16: crosscorr_mfma_pair with 4 or 6 n-tiles; 24: the strip
+    kernel) add each output's products in the same order (template-row pairs in turn, chunks in
+    turn, then the two half sums), so their maps are equal bit for bit, for CrossCorr and for
+    CrossCorr_Norm (the same in-place scale)."""
+    rng = np.random.default_rng(77)
+    src = rng.random((300, 330), dtype=np.float32)
+    tpl = rng.random((40, 52), dtype=np.float32)      # strip schedule fits: 20 steps, KC2 = 5
+    maps = {}
+    for v in (14, 16, 24):
+        monkeypatch.setenv("OCIP_MFMA_VARIANT", str(v))
+        say("row-pair identity", prec, v)
+        maps[v] = (run_op(ocip, "CrossCorr", src, tpl, precision=PREC[prec]),
+                   run_op(ocip, "CrossCorr_Norm", src, tpl, precision=PREC[prec]))
+    for v in (16, 24):
+        np.testing.assert_array_equal(maps[v][0], maps[14][0])
+        np.testing.assert_array_equal(maps[v][1], maps[14][1])
+
+
 @pytest.mark.parametrize("wgs", [1, 3, 7, 0])
 def test_strip_runs(ocip, monkeypatch, wgs):
     """crosscorr_mfma_strip with few workgroups, so that each takes long runs of tiles down its
