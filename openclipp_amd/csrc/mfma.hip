@@ -10,6 +10,7 @@
 #include <functional>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mfma_kernels.h"
@@ -95,9 +96,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NCH <= 3 ? 2
       col[k] = min(max(c, 0), W - 1);
       coff[k] = cok[k] ? c * 4 : kDenOob;
    }
-   const __amdgpu_buffer_rsrc_t rsrc =
-       __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(src), (short)0, BUF ? H * sstep : 0, 0x00020000);
    const int pre = th - 1;                       // steps before the first output row
+   // Rows the band reads: [rlo, rhi).  Every other row reads 0: below rlo (above the image,
+   // steps before the stream, and the lag of steps j < th, which is always below y0 - cy)
+   // through an out-of-range row offset, rhi and beyond (below the image, and steps past the
+   // band's last output row) through the range of the buffer resource.
+   const int rlo = max(0, y0 - cy), rhi = min(H, y0 - cy + pre + n);
+   const __amdgpu_buffer_rsrc_t rsrc =
+       __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(src), (short)0, BUF ? rhi * sstep : 0, 0x00020000);
    // D register sets of batch loads rotate with the batch (D - 1 batches in flight ahead of
    // the one being summed); jstart <= 0 with (pre - jstart) % U == 0 and a multiple b0 of D
    // batches before the first output row, so that every block starts on set 0
@@ -108,19 +114,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NCH <= 3 ? 2
    const int b0 = (pre - jstart) / U;            // first batch that completes output rows
 
    // Every load is unconditional, so a whole batch is in flight at once.  BUF: a buffer load
-   // whose offset (row + column, either kDenOob when outside) is out of range returns 0.
-   // Otherwise a position outside the image (or a step that takes no row) reads the zero of
-   // kZeroPad (an address select, no branch around the load).
+   // whose offset (row + column, either kDenOob when outside) is out of range returns 0; the
+   // row offset takes one scalar select.  Otherwise a position outside the image (or a step
+   // that takes no row) reads the zero of kZeroPad (an address select, no branch around the
+   // load).
    auto load_batch = [&](int bb, float (&L)[U][NCH], float (&G)[U][NCH]) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
          const int j = jstart + bb * U + u;
-         const bool live = j - pre < n;
          const int lr = y0 - cy + j, gr = lr - th;
-         const bool lok = live && j >= 0 && lr >= 0 && lr < H;
-         const bool gok = live && j >= th && gr >= 0 && gr < H;
          if (BUF) {
-            const int loff = lok ? lr * sstep : kDenOob, goff = gok ? gr * sstep : kDenOob;
+            const int loff = lr >= rlo ? lr * sstep : kDenOob, goff = gr >= rlo ? gr * sstep : kDenOob;
 #pragma unroll
             for (int k = 0; k < NCH; ++k) {
                const unsigned lo = (unsigned)coff[k] + (unsigned)loff, go = (unsigned)coff[k] + (unsigned)goff;
@@ -128,6 +132,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NCH <= 3 ? 2
                G[u][k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)go, 0, 0));
             }
          } else {
+            const bool live = j - pre < n;
+            const bool lok = live && j >= 0 && lr >= 0 && lr < H;
+            const bool gok = live && j >= th && gr >= 0 && gr < H;
             const float *lrow = reinterpret_cast<const float *>(src + (size_t)min(max(lr, 0), H - 1) * sstep);
             const float *grow = reinterpret_cast<const float *>(src + (size_t)min(max(gr, 0), H - 1) * sstep);
 #pragma unroll
@@ -196,18 +203,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NCH <= 3 ? 2
       __syncthreads();
    };
 
-   // vertical pass of batch bb: the running sums, and the column sums V of completed output
-   // rows into the LDS block
-   auto process = [&](int bb, const float (&L)[U][NCH], const float (&G)[U][NCH]) {
+   // vertical pass of batch bb: the running sums, and (OUT: batches bb >= b0) the column sums
+   // V of completed output rows into the LDS block.  Before b0 every step is j < th - 1, whose
+   // lag is 0, and completes no output row.  Rows of the last block past the band's end land
+   // in LDS rows that the flush does not read.
+   auto process = [&](auto out, int bb, const float (&L)[U][NCH], const float (&G)[U][NCH]) {
+      constexpr bool OUT = decltype(out)::value;
       const int e0 = (bb - b0) * U;             // output row (band-relative) of step u = 0
 #pragma unroll
       for (int u = 0; u < U; ++u) {
 #pragma unroll
          for (int k = 0; k < NCH; ++k) {
             lead[k] = fma((double)L[u][k], (double)L[u][k], lead[k]);
-            lag[k] = fma((double)G[u][k], (double)G[u][k], lag[k]);
+            if (OUT) lag[k] = fma((double)G[u][k], (double)G[u][k], lag[k]);
          }
-         if (bb >= b0 && e0 + u < n) {
+         if (OUT) {
 #pragma unroll
             for (int k = 0; k < NCH; ++k)
                vbuf[((e0 + u) % kDenRows) * P + 64 * k + lane] = (float)(lead[k] - lag[k]);
@@ -220,20 +230,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NCH <= 3 ? 2
    for (int d = 0; d < D - 1; ++d) load_batch(d, Ls[d], Gs[d]);
    // batches bb .. bb + D - 1 (bb a multiple of D): each step loads batch + D - 1 into the set
    // freed by the previous batch, then sums its own
-   auto rotation = [&](int bb) {
+   auto rotation = [&](auto out, int bb) {
 #pragma unroll
       for (int d = 0; d < D; ++d) {
          load_batch(bb + d + D - 1, Ls[(d + D - 1) % D], Gs[(d + D - 1) % D]);
-         process(bb + d, Ls[d], Gs[d]);
+         process(out, bb + d, Ls[d], Gs[d]);
       }
    };
-   for (int bb = 0; bb < b0; bb += D) rotation(bb);
+   for (int bb = 0; bb < b0; bb += D) rotation(std::false_type(), bb);
    // blocks of kDenRows output rows, BPB (even) batches each
    for (int first = 0; first < n; first += kDenRows) {
       const int nrows = min(kDenRows, n - first);
       load_cval(y0 + first, nrows);
       const int bbase = b0 + first / U;
-      for (int i = 0; i < BPB; i += D) rotation(bbase + i);
+      for (int i = 0; i < BPB; i += D) rotation(std::true_type(), bbase + i);
       flush(y0 + first, nrows);
    }
 }
@@ -511,8 +521,9 @@ static void launch_ncc_scale(hipStream_t s, const View &S, int tw, int th, const
          kernel<<<bands * cblks, 64, lds, s>>>(static_cast<const char *>(S.ptr), S.step, S.width,
                                                S.height, tw, th, tsums, d, dstep, bands);
       };
-      // buffer loads when every byte offset, and two out-of-range offsets added, stay below 2^31
-      const bool buf = (long long)S.height * S.step <= kDenOob;
+      // buffer loads when every row offset the kernel forms (rows -th - 64 .. H + th + 64) and
+      // two out-of-range offsets added stay below 2^31
+      const bool buf = (long long)(S.height + th + 64) * S.step <= kDenOob;
 #define OCIP_DEN_CASE(N) \
    case N: if (buf) go(ncc_den_src<N, true>); else go(ncc_den_src<N, false>); break;
       switch (den_chunks(tw)) {
