@@ -150,18 +150,34 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NCH <= 3 ? 2
 #pragma unroll
    for (int k = 0; k < NCH; ++k) lead[k] = lag[k] = 0.0;
 
-   // the map's values of output rows [ybase, ybase + nrows): lane = column, unconditional loads
-   // (address select), issued before the block's batches so that waiting for them later does
-   // not wait for the newer batch loads
+   // BUF: the map through a buffer resource too, lane = column at byte offset xoff (kDenOob
+   // past the image's width); rows at and below H read 0 and take no store
+   int xoff[kDenCols / 64];
+#pragma unroll
+   for (int hh = 0; hh < kDenCols / 64; ++hh) {
+      const int x = x0 + 64 * hh + lane;
+      xoff[hh] = x < W ? x * 4 : kDenOob;
+   }
+
+   // the map's values of output rows [ybase, ybase + nrows) (BUF: rows up to ybase + kDenRows,
+   // the rows past nrows never stored): lane = column, unconditional loads, issued before the
+   // block's batches so that waiting for them later does not wait for the newer batch loads
    float cval[kDenCols / 64][kDenRows];
    auto load_cval = [&](int ybase, int nrows) {
+      const __amdgpu_buffer_rsrc_t mrsrc =
+          __builtin_amdgcn_make_buffer_rsrc(dst, (short)0, BUF ? H * dstep : 0, 0x00020000);
 #pragma unroll
       for (int hh = 0; hh < kDenCols / 64; ++hh) {
          const int x = x0 + 64 * hh + lane;
 #pragma unroll
          for (int rr = 0; rr < kDenRows; ++rr) {
-            const float *p = reinterpret_cast<const float *>(dst + (size_t)(ybase + rr) * dstep) + x;
-            cval[hh][rr] = *((x < W && rr < nrows) ? p : &kZeroPad);
+            if (BUF) {
+               cval[hh][rr] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                   mrsrc, (int)((unsigned)xoff[hh] + (unsigned)((ybase + rr) * dstep)), 0, 0));
+            } else {
+               const float *p = reinterpret_cast<const float *>(dst + (size_t)(ybase + rr) * dstep) + x;
+               cval[hh][rr] = *((x < W && rr < nrows) ? p : &kZeroPad);
+            }
          }
       }
    };
@@ -175,12 +191,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NCH <= 3 ? 2
          const float *row = vbuf + r * P + xs;
          double hl = 0.0, hg = 0.0;
          for (int i = 0; i < tw - 1; ++i) hl += (double)row[i];
+         // the FP32 reciprocal square root of inv_sqrt for every output, the range that it
+         // needs checked once per lane: a lane with any product outside it runs the same
+         // running sums again with inv_sqrt
+         const double hl0 = hl;
+         double pmin = 1e30, pmax = 1e-30;
 #pragma unroll
          for (int x = 0; x < HALF; ++x) {
             hl += (double)row[x + tw - 1];
             const double ss = hl - hg;
             hg += (double)row[x];
-            res[x] = inv_sqrt(ss * tt);
+            const double p = ss * tt;
+            res[x] = __builtin_amdgcn_rsqf((float)p);
+            pmin = fmin(pmin, p);
+            pmax = fmax(pmax, p);
+         }
+         if (!(pmin >= 1e-30 && pmax <= 1e30)) {
+            hl = hl0;
+            hg = 0.0;
+#pragma unroll
+            for (int x = 0; x < HALF; ++x) {
+               hl += (double)row[x + tw - 1];
+               const double ss = hl - hg;
+               hg += (double)row[x];
+               res[x] = inv_sqrt(ss * tt);
+            }
          }
       }
       __syncthreads();
@@ -189,6 +224,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NCH <= 3 ? 2
          for (int x = 0; x < HALF; ++x) vbuf[r * P + xs + x] = res[x];
       }
       __syncthreads();
+      if (BUF) {
+         // a resource ending at row ybase + nrows drops the stores of the rows past the block
+         const __amdgpu_buffer_rsrc_t srsrc =
+             __builtin_amdgcn_make_buffer_rsrc(dst, (short)0, (ybase + nrows) * dstep, 0x00020000);
+#pragma unroll
+         for (int hh = 0; hh < kDenCols / 64; ++hh) {
+#pragma unroll
+            for (int rr = 0; rr < kDenRows; ++rr)
+               __builtin_amdgcn_raw_buffer_store_b32(
+                   __float_as_uint(cval[hh][rr] * vbuf[rr * P + 64 * hh + lane]), srsrc,
+                   (int)((unsigned)xoff[hh] + (unsigned)((ybase + rr) * dstep)), 0, 0);
+         }
+      } else
 #pragma unroll
       for (int hh = 0; hh < kDenCols / 64; ++hh) {
          const int x = x0 + 64 * hh + lane;
@@ -230,10 +278,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NCH <= 3 ? 2
    for (int d = 0; d < D - 1; ++d) load_batch(d, Ls[d], Gs[d]);
    // batches bb .. bb + D - 1 (bb a multiple of D): each step loads batch + D - 1 into the set
    // freed by the previous batch, then sums its own
+   // (the scheduling barrier keeps each batch's loads ahead of the summing of the batch before
+   // it: left to itself the compiler sinks them below, and every rotation then waits for all
+   // loads in flight)
    auto rotation = [&](auto out, int bb) {
 #pragma unroll
       for (int d = 0; d < D; ++d) {
          load_batch(bb + d + D - 1, Ls[(d + D - 1) % D], Gs[(d + D - 1) % D]);
+         __builtin_amdgcn_sched_barrier(0);
          process(out, bb + d, Ls[d], Gs[d]);
       }
    };
@@ -521,9 +573,11 @@ static void launch_ncc_scale(hipStream_t s, const View &S, int tw, int th, const
          kernel<<<bands * cblks, 64, lds, s>>>(static_cast<const char *>(S.ptr), S.step, S.width,
                                                S.height, tw, th, tsums, d, dstep, bands);
       };
-      // buffer loads when every row offset the kernel forms (rows -th - 64 .. H + th + 64) and
-      // two out-of-range offsets added stay below 2^31
-      const bool buf = (long long)(S.height + th + 64) * S.step <= kDenOob;
+      // buffer loads and stores when every row offset the kernel forms (source rows -th - 64 ..
+      // H + th + 64, map rows up to H + kDenRows) and two out-of-range offsets added stay
+      // below 2^31
+      const bool buf = (long long)(S.height + th + 64) * S.step <= kDenOob &&
+                       (long long)(S.height + kDenRows) * dstep <= kDenOob;
 #define OCIP_DEN_CASE(N) \
    case N: if (buf) go(ncc_den_src<N, true>); else go(ncc_den_src<N, false>); break;
       switch (den_chunks(tw)) {
