@@ -151,7 +151,7 @@ template <> void launch_strip<P>(hipStream_t s, const SingleLaunch &L, int tiles
 template <> void launch_pair_prep<P>(hipStream_t s, const BatchDesc &d)
 {
    const int nfrag = d.nub * pair_frag_steps(d.th, d.nub) * d.KC;
-   pair_prep<P><<<(nfrag + 3) / 4, 256, 0, s>>>(d);
+   pair_prep<P><<<(nfrag + 3) / 4 + 1, 256, 0, s>>>(d);   // + 1: the sum of squares
    launch_check("pair_prep");
 }
 #endif
@@ -159,7 +159,7 @@ template <> void launch_pair_prep<P>(hipStream_t s, const BatchDesc &d)
 template <> void launch_single_prep<P>(hipStream_t s, const BatchDesc &d)
 {
    const int nfrag = d.nub * d.th * d.KC;
-   single_prep<P><<<(nfrag + 3) / 4, 256, 0, s>>>(d);
+   single_prep<P><<<(nfrag + 3) / 4 + 1, 256, 0, s>>>(d);   // + 1: the sum of squares
    launch_check("single_prep");
 }
 

@@ -142,16 +142,17 @@ __device__ __forceinline__ void template_sqsum(const BatchDesc &d, double *red)
    const int pitch = (d.tw + 15) / 16 * 16;
    const int total = pitch * d.th;
    double s2 = 0;
-   for (int k0 = threadIdx.x; k0 < total; k0 += 256 * 8) {
-      float t[8];
+   // thread k sums elements k, k + 256, k + 512, ... in that order, 16 loads in flight
+   for (int k0 = threadIdx.x; k0 < total; k0 += 256 * 16) {
+      float t[16];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
+      for (int i = 0; i < 16; ++i) {
          const int k = k0 + 256 * i;
          const int v = k / pitch, u = k - v * pitch;
          t[i] = (k < total && u < d.tw) ? reinterpret_cast<const float *>(d.tptr + (size_t)v * d.tstep)[u] : 0.f;
       }
 #pragma unroll
-      for (int i = 0; i < 8; ++i) s2 += (double)t[i] * (double)t[i];
+      for (int i = 0; i < 16; ++i) s2 += (double)t[i] * (double)t[i];
    }
    s2 = wave_sum(s2);
    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s2;
@@ -172,19 +173,21 @@ __global__ __launch_bounds__(256) void batch_prep(const BatchDesc *__restrict__ 
    template_sqsum(d, red);
 }
 
-// Single template straight from its image: fragments in every block, the FP64 sum of
-// squares in block 0 (same order as batch_prep).
+// Single template straight from its image: one fragment per wavefront, the FP64 sum of
+// squares (same order as batch_prep) in the last block, alongside.
 template <int P>
 __global__ __launch_bounds__(256) void single_prep(BatchDesc d)
 {
    __shared__ double red[4];
+   if (blockIdx.x == gridDim.x - 1) {
+      template_sqsum(d, red);
+      return;
+   }
    const int nfrag = d.nub * d.th * d.KC;
    const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
    if (f < nfrag)
       build_fragment<P>(d.tptr, d.tstep, d.tw, d.th, d.KC, d.nub, f, threadIdx.x & 63, d.frag0,
                         d.frag1);
-   if (blockIdx.x != 0) return;
-   template_sqsum(d, red);
 }
 
 template <int P>
@@ -481,19 +484,21 @@ __device__ __forceinline__ void build_pair_fragment(const char *tptr, int tstep,
    frag[(size_t)f * 64 + lane] = h;
 }
 
-// Fragments of the row-pair kernel in every block, the FP64 sum of squares in block 0 (same
-// order as single_prep).
+// Fragments of the row-pair kernel, one per wavefront, the FP64 sum of squares (same order
+// as single_prep) in the last block, alongside.
 template <int P>
 __global__ __launch_bounds__(256) void pair_prep(BatchDesc d)
 {
    __shared__ double red[4];
+   if (blockIdx.x == gridDim.x - 1) {
+      template_sqsum(d, red);
+      return;
+   }
    const int nfrag = d.nub * pair_frag_steps(d.th, d.nub) * d.KC;
    const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
    if (f < nfrag)
       build_pair_fragment<P>(d.tptr, d.tstep, d.tw, d.th, d.KC, d.nub, f, threadIdx.x & 63,
                              d.frag0);
-   if (blockIdx.x != 0) return;
-   template_sqsum(d, red);
 }
 
 // One step: NT + KC - 1 A fragments from the LDS row `ab`, KC B fragments.
