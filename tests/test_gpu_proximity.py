@@ -222,6 +222,26 @@ def test_nan_on_empty_window(ocip):
 
 
 # ------------------------------------------------------------------------ full-size workloads
+@pytest.mark.parametrize("dtype", [np.float32, np.uint8])
+def test_mid_size_bands(ocip, dtype):
+    """1536 x 1408 with a 16 x 16 template: the tile of 8 outputs per thread (images this
+    size keep >= 1024 workgroups with it; smaller ones take the 4-output tile), checked on
+    bands of rows against the oracle, bit-exact."""
+    rng = np.random.default_rng(1536)
+    src = rand(rng, (1408, 1536), dtype)
+    tpl = rand(rng, (16, 16), dtype)
+    for op in RAW_OPS:
+        say("mid-size", np.dtype(dtype).name, op)
+        got = run_op(ocip, op, src, tpl)
+        if dtype == np.float32:
+            fn = lambda c: orc.prox_ref(OP_CODE[op], c, tpl, contract=True)
+        else:
+            fn = lambda c: orc.f64_from_i64(OP_CODE[op], c, tpl).astype(np.float32)
+        for y0, y1 in [(0, 9), (700, 706), (1400, 1408)]:
+            np.testing.assert_array_equal(got[y0:y1], band_oracle(fn, src, tpl, y0, y1),
+                                          err_msg=op)
+
+
 def test_plain_1024_invariants(ocip):
     from openclipp_amd import workloads
     src, tpl, (ox, oy) = workloads.plain_1024()
