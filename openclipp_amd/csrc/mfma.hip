@@ -48,10 +48,11 @@ __global__ void reset_counters(unsigned *c, int n)
 //    (loaded before the horizontal pass, so that their latency overlaps it).
 // The rows are a stream of steps: step j takes row y0 - th/2 + j into `lead` (j >= 0), row
 // y0 - th/2 + j - th into `lag` (j >= th) and completes output row y0 + j - (th - 1).  Steps
-// run in batches of U, the loads of batch b + 1 in flight while batch b is summed; the
-// stream starts at jstart <= 0 (steps before 0 load nothing) so that output rows fill the
-// LDS block at batch boundaries.  No integral is formed: the source is read from HBM about
-// once (halo columns and the lag rows come from L2) and the map is read and written once.
+// run in batches of U, the loads of the next D - 1 batches in flight while a batch is summed;
+// the stream starts at jstart <= 0 (steps before 0 load nothing) so that output rows fill the
+// LDS block at batch boundaries.  No integral is formed: the source is read once from HBM or
+// the caches (the halo columns and the lag rows are read a second time) and the map is read
+// and written once.
 constexpr int kDenRows = 32;     // output rows per LDS block
 constexpr int kDenBand = 128;    // output rows per wavefront
 constexpr int kDenCols = 64;     // output columns per wavefront (two halves of 32)
