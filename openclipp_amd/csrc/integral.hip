@@ -116,16 +116,18 @@ __global__ __launch_bounds__(kThreads) void integral_aggregate(const char *src, 
    if (threadIdx.x == 0) aux.tiletot[(size_t)rb * aux.CB + cb] = tot;
 }
 
-// exclusive scan in place of n values at p[0], p[stride], ... (loads issued 8 at a time)
+// exclusive scan in place of n values at p[0], p[stride], ...
 __device__ __forceinline__ void exclusive_scan_strided(double *p, int n, size_t stride)
 {
+   // 32 loads in flight: the scans are chains of round trips (128 tile rows at 4096 rows)
+   constexpr int B = 32;
    double acc = 0;
-   for (int k = 0; k < n; k += 8) {
-      double v[8];
+   for (int k = 0; k < n; k += B) {
+      double v[B];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) v[i] = k + i < n ? p[(size_t)(k + i) * stride] : 0.0;
+      for (int i = 0; i < B; ++i) v[i] = k + i < n ? p[(size_t)(k + i) * stride] : 0.0;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
+      for (int i = 0; i < B; ++i) {
          if (k + i < n) p[(size_t)(k + i) * stride] = acc;
          acc += v[i];
       }
