@@ -34,7 +34,8 @@ def bound(prec, src, tpl):
     return (UNIT[prec] + 2 * (n + 16) * 2.0 ** -24) * sabs + 1e-30
 
 
-TEMPLATES = [(1, 1), (3, 5), (16, 16), (31, 31), (64, 64), (96, 40), (17, 64), (100, 9), (130, 3)]
+TEMPLATES = [(1, 1), (3, 5), (16, 16), (31, 31), (64, 64), (96, 40), (17, 64), (100, 9), (130, 3),
+             (132, 5), (260, 3)]
 
 
 @pytest.mark.parametrize("prec", list(PREC))
@@ -105,6 +106,19 @@ def test_norm_zero_windows(ocip, prec):
     np.testing.assert_array_equal(np.isnan(got), np.isnan(ncc64))
     ok = np.isfinite(ncc64)
     assert np.max(np.abs(got - ncc64)[ok]) <= 2 * UNIT[prec] + 1e-6
+
+
+@pytest.mark.parametrize("W,tw", [(460, 388), (461, 388), (301, 132)])
+def test_norm_wide_windows(ocip, W, tw):
+    """The denominator pass with window segments of 8 (388 columns) and 4 (132) chunks of 64
+    columns, on even and odd image widths."""
+    rng = np.random.default_rng(W + tw)
+    src = rng.random((40, W), dtype=np.float32)
+    tpl = rng.random((3, tw), dtype=np.float32)
+    say("wide windows", W, tw)
+    ncc64 = orc.prox_f64(orc.CC_NORM, src, tpl)
+    got = run_op(ocip, "CrossCorr_Norm", src, tpl, precision=1).astype(np.float64)
+    assert np.max(np.abs(got - ncc64)) <= 2 * UNIT["bf16"] + 1e-6
 
 
 def test_norm_of_an_image_over_one_gib(ocip):
