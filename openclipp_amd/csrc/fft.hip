@@ -12,7 +12,7 @@
 // zero border.
 //
 // This build computes in FP64 complex arithmetic (the reference uses clFFT in single
-// precision and an F32 integral) with its own mixed-radix (2, 3, 4, 5) Stockham FFT in LDS:
+// precision and an F32 integral) with its own mixed-radix (2, 3, 4, 5, 8, 9) Stockham FFT in LDS:
 //   pass 1 (fft_rows_pack): one workgroup per row y of the packed image z = s + i t (source
 //          and template share one complex transform), FFT along x, written transposed;
 //   pass 2 (fft_cols_pair): one workgroup per pair of spectrum columns (kx, -kx): FFT along y
@@ -120,6 +120,53 @@ template <> __device__ __forceinline__ void dft<5>(double2 (&u)[5])
    u[3] = cadd(m2, mul_pi(n2));
 }
 
+// 8 points: DFT4 of the even and of the odd inputs, the odd ones times w8^k = exp(-2 pi i k/8)
+template <> __device__ __forceinline__ void dft<8>(double2 (&u)[8])
+{
+   const double r = 0.70710678118654752440;  // 1 / sqrt(2)
+   double2 e[4] = {u[0], u[2], u[4], u[6]}, o[4] = {u[1], u[3], u[5], u[7]};
+   dft<4>(e);
+   dft<4>(o);
+   const double2 t[4] = {o[0], make_double2(r * (o[1].x + o[1].y), r * (o[1].y - o[1].x)),
+                         mul_mi(o[2]), make_double2(r * (o[3].y - o[3].x), -r * (o[3].x + o[3].y))};
+#pragma unroll
+   for (int k = 0; k < 4; ++k) {
+      u[k] = cadd(e[k], t[k]);
+      u[k + 4] = csub(e[k], t[k]);
+   }
+}
+
+// 9 points: DFT3 over m of x[3m + p] for p = 0, 1, 2, twiddles w9^(p k1), DFT3 over p:
+// X[k1 + 3 k2]
+template <> __device__ __forceinline__ void dft<9>(double2 (&u)[9])
+{
+   // w9^1, w9^2, w9^4 with w9 = exp(-2 pi i / 9)
+   const double2 w1 = make_double2(0.76604444311897803520, -0.64278760968653932632);
+   const double2 w2 = make_double2(0.17364817766693034885, -0.98480775301220805936);
+   const double2 w4 = make_double2(-0.93969262078590838405, -0.34202014332566873304);
+   double2 a[3][3];
+#pragma unroll
+   for (int p = 0; p < 3; ++p) {
+      double2 v[3] = {u[p], u[p + 3], u[p + 6]};
+      dft<3>(v);
+      a[p][0] = v[0];
+      a[p][1] = v[1];
+      a[p][2] = v[2];
+   }
+   a[1][1] = cmul(a[1][1], w1);
+   a[1][2] = cmul(a[1][2], w2);
+   a[2][1] = cmul(a[2][1], w2);
+   a[2][2] = cmul(a[2][2], w4);
+#pragma unroll
+   for (int k1 = 0; k1 < 3; ++k1) {
+      double2 v[3] = {a[0][k1], a[1][k1], a[2][k1]};
+      dft<3>(v);
+      u[k1] = v[0];
+      u[k1 + 3] = v[1];
+      u[k1 + 6] = v[2];
+   }
+}
+
 // Workgroup barrier that orders LDS only (a full __syncthreads also waits for the
 // workgroup's outstanding global loads and stores, which the FFT stages do not need).
 __device__ __forceinline__ void lds_barrier()
@@ -181,6 +228,9 @@ __device__ __forceinline__ void fft_stage(double2 *x, int n, int nseq, int span,
 
 // Forward FFT (unnormalised) of nseq sequences of length P.n in LDS, natural order in and out;
 // tw: the plan's twiddles, in LDS.
+// LARGE: the plan may hold radix-8 and radix-9 stages (row plans only: the column kernels,
+// which hold two columns, have no registers to spare for them)
+template <bool LARGE = false>
 __device__ __forceinline__ void lds_fft(double2 *x, int nseq, const FftPlan *__restrict__ P, const double2 *tw)
 {
    const int n = P->n, ns = P->nstages;
@@ -191,6 +241,12 @@ __device__ __forceinline__ void lds_fft(double2 *x, int nseq, const FftPlan *__r
       case 2: fft_stage<2>(x, n, nseq, span, t); break;
       case 3: fft_stage<3>(x, n, nseq, span, t); break;
       case 4: fft_stage<4>(x, n, nseq, span, t); break;
+      case 8:
+         if constexpr (LARGE) fft_stage<8>(x, n, nseq, span, t);
+         break;
+      case 9:
+         if constexpr (LARGE) fft_stage<9>(x, n, nseq, span, t);
+         break;
       default: fft_stage<5>(x, n, nseq, span, t); break;
       }
    }
@@ -227,7 +283,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_rows_pack(const char *__restr
       fbuf[x] = make_double2(sv, tv);
    }
    lds_barrier();
-   lds_fft(fbuf, 1, pw, ltw);
+   lds_fft<true>(fbuf, 1, pw, ltw);
    for (int x = threadIdx.x; x < Wb; x += kFftThreads) out[(size_t)x * Hb + y] = fbuf[x];
 }
 
@@ -301,7 +357,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_rows_epi(const double2 *__res
       const int y = blockIdx.x;
       for (int x = threadIdx.x; x < Wb; x += kFftThreads) fbuf[x] = in[(size_t)y * Wb + x];
       lds_barrier();
-      lds_fft(fbuf, 1, pw, ltw);
+      lds_fft<true>(fbuf, 1, pw, ltw);
       float *drow = reinterpret_cast<float *>(dst + (size_t)y * dstep);
       for (int x = threadIdx.x; x < W; x += kFftThreads) {
          const double cc = fbuf[x].x;
@@ -336,7 +392,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_r2c_rows(const char *__restri
    const T *row = reinterpret_cast<const T *>(src + (size_t)y * sstep);
    for (int x = threadIdx.x; x < W; x += kFftThreads) fbuf[x] = make_double2((double)row[x], 0.0);
    lds_barrier();
-   lds_fft(fbuf, 1, pw, ltw);
+   lds_fft<true>(fbuf, 1, pw, ltw);
    for (int kx = threadIdx.x; kx <= W / 2; kx += kFftThreads) work[(size_t)kx * H + y] = fbuf[kx];
 }
 
@@ -397,16 +453,17 @@ __global__ __launch_bounds__(kFftThreads) void fft_c2r_rows(const double2 *__res
    for (int kx = threadIdx.x; kx < W; kx += kFftThreads)
       fbuf[kx] = kx <= W / 2 ? work[(size_t)kx * H + y] : cconj(work[(size_t)(W - kx) * H + y]);
    lds_barrier();
-   lds_fft(fbuf, 1, pw, ltw);
+   lds_fft<true>(fbuf, 1, pw, ltw);
    T *row = reinterpret_cast<T *>(dst + (size_t)y * dstep);
    for (int x = threadIdx.x; x < W; x += kFftThreads) row[x] = (T)fbuf[x].x;
 }
 
 
 // Stages for a 2^a 3^b 5^c length, larger radices last (the twiddle table of a stage has span
-// entries, so this keeps the table small); twiddles appended to `table`, offsets relative to
-// the plan's first entry.
-void make_plan(int n, FftPlan &p, std::vector<double2> &table)
+// entries, so this keeps the table small): LARGE (row plans), radix 8 and 9 where they fit
+// (fewer passes through LDS) and 4 / 2 and 3 for what is left; otherwise radices 2 to 5.
+// Twiddles appended to `table`, offsets relative to the plan's first entry.
+void make_plan(int n, FftPlan &p, std::vector<double2> &table, bool large)
 {
    p.n = n;
    p.nstages = 0;
@@ -430,10 +487,34 @@ void make_plan(int n, FftPlan &p, std::vector<double2> &table)
       m /= 2;
       ++twos;
    }
-   if (twos % 2) add(2);
-   while (rest % 3 == 0) add(3);
-   while (rest % 4 == 0) add(4);
+   if (!large) {
+      if (twos % 2) add(2);
+      while (rest % 3 == 0) add(3);
+      while (rest % 4 == 0) add(4);
+      while (rest % 5 == 0) add(5);
+      if (rest != 1) fail(CL_INVALID_IMAGE_SIZE, "FFT length with a prime factor above 5");
+      p.twsize = (int)table.size() - base;
+      return;
+   }
+   int threes = 0;
+   for (int q = m; q % 3 == 0; q /= 3) ++threes;
+   // radix 8 and 9 where they fit (fewer LDS passes), 4 / 2 and 3 for the rest
+   if (twos % 3 == 1 && twos >= 4) {
+      add(4);
+      add(4);
+      twos -= 4;
+   } else if (twos % 3 == 1) {
+      add(2);
+      twos -= 1;
+   } else if (twos % 3 == 2) {
+      add(4);
+      twos -= 2;
+   }
+   const int nines = threes / 2;
+   for (int k = threes - 2 * nines; k > 0; --k) add(3);
    while (rest % 5 == 0) add(5);
+   for (; twos > 0; twos -= 3) add(8);
+   for (int k = nines; k > 0; --k) add(9);
    if (rest != 1) fail(CL_INVALID_IMAGE_SIZE, "FFT length with a prime factor above 5");
    p.twsize = (int)table.size() - base;
 }
@@ -486,8 +567,8 @@ cl_int fft_sizes(int W, int H, int tw, int th, int *Wb, int *Hb)
    if (*Wb > kFftMaxLen || 2 * *Hb > kFftMaxLen) return CL_INVALID_IMAGE_SIZE;
    std::vector<double2> table;
    FftPlan pw, ph;
-   make_plan(*Wb, pw, table);
-   make_plan(*Hb, ph, table);
+   make_plan(*Wb, pw, table, true);
+   make_plan(*Hb, ph, table, false);
    if (row_lds(pw) > (size_t)kFftMaxLen * sizeof(double2) ||
        pair_lds(ph) > (size_t)kFftMaxLen * sizeof(double2))
       return CL_INVALID_IMAGE_SIZE;
@@ -501,9 +582,9 @@ void fft_prepare(hipStream_t s, FftState *st, int W, int H, int tw, int th)
    if (fft_sizes(W, H, tw, th, &Wb, &Hb) != CL_SUCCESS) fail(CL_INVALID_IMAGE_SIZE, "image too big for the FFT path");
    if (st->Wb == Wb && st->Hb == Hb) return;
    std::vector<double2> table;  // the width plan's twiddles, then the height plan's
-   make_plan(Wb, st->pw, table);
+   make_plan(Wb, st->pw, table, true);
    st->hbase = (int)table.size();
-   make_plan(Hb, st->ph, table);
+   make_plan(Hb, st->ph, table, false);
    st->twid.ensure(table.size() * sizeof(double2));
    hip_check(hipMemcpyAsync(st->twid.ptr, table.data(), table.size() * sizeof(double2),
                             hipMemcpyHostToDevice, s), "twiddles");
@@ -564,8 +645,8 @@ cl_int fft_plain_sizes(int W, int H)
    if (!fft_supported_length(W) || !fft_supported_length(H)) return CL_INVALID_IMAGE_SIZE;
    std::vector<double2> table;
    FftPlan pw, ph;
-   make_plan(W, pw, table);
-   make_plan(H, ph, table);
+   make_plan(W, pw, table, true);
+   make_plan(H, ph, table, false);
    if (row_lds(pw) > (size_t)kFftMaxLen * sizeof(double2) ||
        row_lds(ph) > (size_t)kFftMaxLen * sizeof(double2))
       return CL_INVALID_IMAGE_SIZE;
@@ -578,9 +659,9 @@ static void fft_prepare_plain(hipStream_t s, FftState *st, int W, int H)
    if (fft_plain_sizes(W, H) != CL_SUCCESS) fail(CL_INVALID_IMAGE_SIZE, "FFT size");
    if (st->Wb == W && st->Hb == H) return;
    std::vector<double2> table;
-   make_plan(W, st->pw, table);
+   make_plan(W, st->pw, table, true);
    st->hbase = (int)table.size();
-   make_plan(H, st->ph, table);
+   make_plan(H, st->ph, table, false);
    st->twid.ensure(table.size() * sizeof(double2));
    hip_check(hipMemcpyAsync(st->twid.ptr, table.data(), table.size() * sizeof(double2),
                             hipMemcpyHostToDevice, s), "twiddles");

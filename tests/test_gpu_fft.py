@@ -170,7 +170,8 @@ def complex_image(ocip, H, W, dtype, extra=0):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("shape", [(1, 1), (8, 12), (30, 45), (64, 100), (27, 250), (81, 2)])
+@pytest.mark.parametrize("shape", [(1, 1), (8, 12), (30, 45), (64, 100), (27, 250), (81, 2),
+                                   (16, 64), (9, 72)])
 def test_fft_forward_inverse_against_numpy(ocip, dtype, shape):
     """Forward = numpy.fft.rfft2 (unnormalised half spectrum along x), inverse =
     W*H*irfft2; FP64 inside, so F64 agrees to ~1e-13 of the sum of |x| and F32 to its own
