@@ -85,11 +85,14 @@ __global__ __launch_bounds__(kThreads) void integral_aggregate(const char *src, 
                                                                 int step, int nch, int ch,
                                                                 IntegralAux aux)
 {
+   // each lane's partial row sums, rows x lanes (odd pitch: lane r's walk along row r
+   // spreads over the banks); lane r then sums row r in lane order
+   __shared__ double part[kRH][kThreads + 1];
+   static_assert(kRH % kRowUnroll == 0 && kRH <= kThreads, "rows of the tile");
    const int cb = blockIdx.x, rb = blockIdx.y;
    const int x0 = cb * kCW + threadIdx.x * kVec;
    const int y0 = rb * kRH, y1 = min(y0 + kRH, H);
    double col[kVec] = {0, 0, 0, 0};
-   double tot = 0;
    for (int y = y0; y < y1; y += kRowUnroll) {
       double v[kRowUnroll][kVec];
 #pragma unroll
@@ -103,16 +106,20 @@ __global__ __launch_bounds__(kThreads) void integral_aggregate(const char *src, 
             col[j] += v[r][j];
             rs += v[r][j];
          }
-         rs = wave_sum(rs);
-         if (y + r < y1) {
-            if (threadIdx.x == 0) aux.rowsum[(size_t)(y + r) * aux.CB + cb] = rs;
-            tot += rs;
-         }
+         part[y - y0 + r][threadIdx.x] = rs;   // rows past y1 hold 0 and are not summed
       }
    }
 #pragma unroll
    for (int j = 0; j < kVec; ++j)
       if (x0 + j < W) aux.colsum[(size_t)rb * W + x0 + j] = col[j];
+   __syncthreads();
+   const int r = threadIdx.x;
+   double rs = 0;
+   if (r < y1 - y0) {
+      for (int l = 0; l < kThreads; ++l) rs += part[r][l];
+      aux.rowsum[(size_t)(y0 + r) * aux.CB + cb] = rs;
+   }
+   const double tot = wave_sum(rs);
    if (threadIdx.x == 0) aux.tiletot[(size_t)rb * aux.CB + cb] = tot;
 }
 
