@@ -552,6 +552,9 @@ void crosscorr_mfma_pair(TileArgs a, int ldsp, int tiles_x, int tiles)
                               kTileRows - 2 + 2 * ns);
          __syncthreads();
 
+         // a wave whose output rows (from Y0 - 1 + 32 wave on) all lie below the image skips
+         // its products: its half rows are never stored
+         if (Y0 - 1 + 32 * wave >= a.H) continue;
          // ping-pong: the B fragments of step s+1 load while step s multiplies
          const size_t fb0 = ((size_t)(ub * pair_frag_steps(a.th, a.nub) + s0) * KC) * 64 + lane;
          u16x8 pa[KC], pb[KC], dummy[KC];
