@@ -599,13 +599,33 @@ void crosscorr_mfma_pair(TileArgs a, int ldsp, int tiles_x, int tiles)
          }
    }
    __syncthreads();
-   // rows 1..127 are whole; consecutive threads take consecutive columns
-   for (int k = threadIdx.x; k < kPairRows * C; k += 256) {
-      const int r = k / C, c = k - r * C;
-      const int y = Y0 + r, x = x0 + c;
-      if (y >= a.H || x >= a.W) continue;
-      reinterpret_cast<float *>(reinterpret_cast<char *>(a.dst) + (size_t)y * a.dstep)[x] =
-          tile[(r + 1) * TP + c];
+   // rows 1..127 are whole.  16-byte aligned output rows: four consecutive columns per
+   // thread (the tile pitch keeps them 16-byte aligned in LDS too), one at a time at the
+   // right edge; otherwise consecutive threads take consecutive columns
+   static_assert(TP % 4 == 0 && C % 4 == 0, "16-byte groups of the LDS tile");
+   if (a.dstep % 16 == 0 && reinterpret_cast<uintptr_t>(a.dst) % 16 == 0) {
+      for (int k = threadIdx.x; k < kPairRows * (C / 4); k += 256) {
+         const int r = k / (C / 4), c = 4 * (k - r * (C / 4));
+         const int y = Y0 + r, x = x0 + c;
+         if (y >= a.H || x >= a.W) continue;
+         float *o = reinterpret_cast<float *>(reinterpret_cast<char *>(a.dst) + (size_t)y * a.dstep) + x;
+         const float4 q = *reinterpret_cast<const float4 *>(&tile[(r + 1) * TP + c]);
+         if (x + 4 <= a.W) {
+            *reinterpret_cast<float4 *>(o) = q;
+         } else {
+            o[0] = q.x;
+            if (x + 1 < a.W) o[1] = q.y;
+            if (x + 2 < a.W) o[2] = q.z;
+         }
+      }
+   } else {
+      for (int k = threadIdx.x; k < kPairRows * C; k += 256) {
+         const int r = k / C, c = k - r * C;
+         const int y = Y0 + r, x = x0 + c;
+         if (y >= a.H || x >= a.W) continue;
+         reinterpret_cast<float *>(reinterpret_cast<char *>(a.dst) + (size_t)y * a.dstep)[x] =
+             tile[(r + 1) * TP + c];
+      }
    }
 }
 
