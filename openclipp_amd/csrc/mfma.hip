@@ -67,7 +67,7 @@ size_t den_lds_bytes(int tw) { return (size_t)kDenRows * (den_chunks(tw) * 64 + 
 // outside the image when the image holds fewer than kDenOob bytes.
 constexpr int kDenOob = 0x40000000;
 
-template <int NCH, bool BUF>
+template <int NCH, bool BUF, bool V4>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NCH <= 3 ? 2 : 1, NCH <= 6 ? 2 : 1))) void ncc_den_src(const char *__restrict__ src, int sstep, int W,
                                                   int H, int tw, int th,
                                                   const double *__restrict__ tsums,
@@ -159,6 +159,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NCH <= 3 ? 2
       const int x = x0 + 64 * hh + lane;
       xoff[hh] = x < W ? x * 4 : kDenOob;
    }
+   // V4 (BUF, W a multiple of 4, 16-byte aligned map rows): lane = 4 columns 4 g4 .. 4 g4 + 3
+   // of rows r4, r4 + 4, ..., 16-byte loads and stores
+   static_assert(!V4 || (BUF && kDenCols == 64), "16-byte map access: buffer path, 64 columns");
+   const int g4 = lane & 15, r4 = lane >> 4;
+   const int xoff4 = x0 + 4 * g4 < W ? (x0 + 4 * g4) * 4 : kDenOob;
 
    // the map's values of output rows [ybase, ybase + nrows) (BUF: rows up to ybase + kDenRows,
    // the rows past nrows never stored): lane = column, unconditional loads, issued before the
@@ -167,6 +172,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NCH <= 3 ? 2
    auto load_cval = [&](int ybase, int nrows) {
       const __amdgpu_buffer_rsrc_t mrsrc =
           __builtin_amdgcn_make_buffer_rsrc(dst, (short)0, BUF ? H * dstep : 0, 0x00020000);
+      if (V4) {
+#pragma unroll
+         for (int k = 0; k < kDenRows / 4; ++k) {
+            const auto q = __builtin_amdgcn_raw_buffer_load_b128(
+                mrsrc, (int)((unsigned)xoff4 + (unsigned)((ybase + r4 + 4 * k) * dstep)), 0, 0);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) cval[0][4 * k + e] = __uint_as_float(q[e]);
+         }
+         return;
+      }
 #pragma unroll
       for (int hh = 0; hh < kDenCols / 64; ++hh) {
          const int x = x0 + 64 * hh + lane;
@@ -225,7 +240,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NCH <= 3 ? 2
          for (int x = 0; x < HALF; ++x) vbuf[r * P + xs + x] = res[x];
       }
       __syncthreads();
-      if (BUF) {
+      if (V4) {
+         const __amdgpu_buffer_rsrc_t srsrc =
+             __builtin_amdgcn_make_buffer_rsrc(dst, (short)0, (ybase + nrows) * dstep, 0x00020000);
+#pragma unroll
+         for (int k = 0; k < kDenRows / 4; ++k) {
+            const int rr = r4 + 4 * k;
+            __attribute__((ext_vector_type(4))) unsigned q;
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+               q[e] = __float_as_uint(cval[0][4 * k + e] * vbuf[rr * P + 4 * g4 + e]);
+            __builtin_amdgcn_raw_buffer_store_b128(
+                q, srsrc, (int)((unsigned)xoff4 + (unsigned)((ybase + rr) * dstep)), 0, 0);
+         }
+      } else if (BUF) {
          // a resource ending at row ybase + nrows drops the stores of the rows past the block
          const __amdgpu_buffer_rsrc_t srsrc =
              __builtin_amdgcn_make_buffer_rsrc(dst, (short)0, (ybase + nrows) * dstep, 0x00020000);
@@ -583,8 +611,12 @@ static void launch_ncc_scale(hipStream_t s, const View &S, int tw, int th, const
       // below 2^31
       const bool buf = (long long)(S.height + th + 64) * S.step <= kDenOob &&
                        (long long)(S.height + kDenRows) * dstep <= kDenOob;
+      // 16-byte map loads and stores: whole 4-column groups, 16-byte aligned rows
+      const bool v4 = buf && S.width % 4 == 0 && dstep % 16 == 0 &&
+                      reinterpret_cast<uintptr_t>(d) % 16 == 0;
 #define OCIP_DEN_CASE(N) \
-   case N: if (buf) go(ncc_den_src<N, true>); else go(ncc_den_src<N, false>); break;
+   case N: if (v4) go(ncc_den_src<N, true, true>); else if (buf) go(ncc_den_src<N, true, false>); \
+           else go(ncc_den_src<N, false, false>); break;
       switch (den_chunks(tw)) {
       OCIP_DEN_CASE(1)
       OCIP_DEN_CASE(2)
@@ -594,7 +626,8 @@ static void launch_ncc_scale(hipStream_t s, const View &S, int tw, int th, const
       OCIP_DEN_CASE(6)
       OCIP_DEN_CASE(7)
       OCIP_DEN_CASE(8)
-      default: if (buf) go(ncc_den_src<9, true>); else go(ncc_den_src<9, false>); break;
+      default: if (v4) go(ncc_den_src<9, true, true>); else if (buf) go(ncc_den_src<9, true, false>);
+               else go(ncc_den_src<9, false, false>); break;
       }
 #undef OCIP_DEN_CASE
    } else {
