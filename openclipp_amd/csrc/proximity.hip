@@ -206,12 +206,32 @@ __device__ __forceinline__ void stage_tile_int(int32_t *lds, int ldsw, int nrows
    }
 }
 
+// One tap.  int32 sums come only from 8-bit sources, whose values and differences fit the
+// 24-bit multiplies (v_mad_i32_i24) and, for SAD, v_sad_u16 (the absolute differences of
+// both 16-bit halves plus c; its operands are kept in 0..255, see sad_key, so the high
+// halves are zero).
 template <int OP, class ACC> __device__ __forceinline__ ACC itap(ACC acc, int32_t s, int32_t t)
 {
-   if (OP == kSSD || OP == kSSDNorm) { int32_t d = s - t; return acc + (ACC)d * (ACC)d; }
-   if (OP == kSAD) { int32_t d = s - t; return acc + (ACC)(d < 0 ? -d : d); }
-   return acc + (ACC)s * (ACC)t;
+   if constexpr (sizeof(ACC) == 4) {
+      if (OP == kSSD || OP == kSSDNorm) { int32_t d = s - t; return acc + __mul24(d, d); }
+      if (OP == kSAD) return (ACC)__builtin_amdgcn_sad_u16((uint32_t)s, (uint32_t)t, (uint32_t)acc);
+      return acc + __mul24(s, t);
+   } else {
+      if (OP == kSSD || OP == kSSDNorm) { int32_t d = s - t; return acc + (ACC)d * (ACC)d; }
+      if (OP == kSAD) { int32_t d = s - t; return acc + (ACC)(d < 0 ? -d : d); }
+      return acc + (ACC)s * (ACC)t;
+   }
 }
+
+// Operand of itap: for SAD with int32 sums on a signed 8-bit source, the value + 128, which
+// maps -128..127 in order onto 0..255 (a sign-extended negative value would put 0xffff in
+// v_sad_u16's high half); any other value unchanged.
+template <int OP, int TYPE, class ACC> __device__ __forceinline__ int32_t sad_key(int32_t v)
+{
+   return (OP == kSAD && sizeof(ACC) == 4 && TYPE == SImage::S8) ? v + 128 : v;
+}
+
+constexpr int kIntU = 8;   // template columns per unrolled chunk of prox_tile_int
 
 template <int OP, int TYPE, int R, class ACC>
 __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int H, int sstep,
@@ -242,17 +262,26 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
       for (int v = 0; v < nv; ++v) {
          const int32_t *lrow = ldsi + (size_t)(lane + v) * ldsw + colbase;
          const int32_t *trow = tp + (size_t)(v0 + v) * tpitch;
-         // sliding window over the template row, one column at a time
-         int32_t s[R];
+         // the template row in chunks of kIntU columns: kIntU template values and the
+         // R + kIntU - 1 window values they meet, then R x kIntU taps (exact integer sums,
+         // so the order is free); LDS columns read: colbase .. colbase + R + tw - 2
+         int u0 = 0;
+         for (; u0 + kIntU <= tw; u0 += kIntU) {
+            int32_t t[kIntU], sw[R + kIntU - 1];
 #pragma unroll
-         for (int r = 0; r < R; ++r) s[r] = lrow[r];
-         for (int u = 0; u < tw; ++u) {
-            const int32_t t = trow[u];
+            for (int k = 0; k < kIntU; ++k) t[k] = sad_key<OP, TYPE, ACC>(trow[u0 + k]);
 #pragma unroll
-            for (int r = 0; r < R; ++r) acc[r] = itap<OP, ACC>(acc[r], s[r], t);
+            for (int j = 0; j < R + kIntU - 1; ++j) sw[j] = sad_key<OP, TYPE, ACC>(lrow[u0 + j]);
 #pragma unroll
-            for (int r = 0; r < R - 1; ++r) s[r] = s[r + 1];
-            s[R - 1] = lrow[R + u];
+            for (int k = 0; k < kIntU; ++k)
+#pragma unroll
+               for (int r = 0; r < R; ++r) acc[r] = itap<OP, ACC>(acc[r], sw[r + k], t[k]);
+         }
+         for (; u0 < tw; ++u0) {
+            const int32_t t = sad_key<OP, TYPE, ACC>(trow[u0]);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+               acc[r] = itap<OP, ACC>(acc[r], sad_key<OP, TYPE, ACC>(lrow[u0 + r]), t);
          }
       }
    }
