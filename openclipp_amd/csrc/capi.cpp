@@ -164,10 +164,13 @@ void run_proximity(int op, ocipImage hs, ocipImage ht, ocipImage hd, int precisi
    c.tmpl.data.ensure((size_t)tpitch * (th > 0 ? th : 1) * esz);
    c.tmpl.sums.ensure(2 * sizeof(double));
 
+   // SqrDistance on 8-bit sources is formed from the cross term and the window sums of
+   // squares, which come from the integral as for the Norm ops
+   const bool ssd8 = op == kSSD && (type == SImage::U8 || type == SImage::S8);
    for (int ch = 0; ch < nch; ++ch) {
       const bool mfma = precision != ocipPrecisionF32;
-      const double *integ =
-          norm && (!mfma || crosscorr_mfma_needs_integral(tw)) ? sqr_integral_for(c, S, ch) : nullptr;
+      const bool need_integ = norm ? !mfma || crosscorr_mfma_needs_integral(tw) : ssd8 && !mfma;
+      const double *integ = need_integ ? sqr_integral_for(c, S, ch) : nullptr;
       if (mfma) {
          size_t wb = crosscorr_mfma_work_bytes(precision, S.desc.Width, S.desc.Height, tw, th);
          c.stats.ensure(wb > 0 ? wb : 1);

@@ -233,6 +233,14 @@ template <int OP, int TYPE, class ACC> __device__ __forceinline__ int32_t sad_ke
 
 constexpr int kIntU = 8;   // template columns per unrolled chunk of prox_tile_int
 
+// With int32 sums (8-bit sources) SqrDistance and SqrDistance_Norm take the cross term only,
+// one multiply-add per tap, and finish as sum s^2 - 2 sum s*t + sum t^2: the window's sum of
+// squares from the FP64 integral (an exact integer below 2^53), sum t^2 from tsums[0].
+template <int OP, class ACC> constexpr bool ssd_via_cc()
+{
+   return (OP == kSSD || OP == kSSDNorm) && sizeof(ACC) == 4;
+}
+
 template <int OP, int TYPE, int R, class ACC>
 __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int H, int sstep,
                                                       int nch, int ch,
@@ -243,6 +251,8 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
                                                       char *dst, int dstep, int vb, int ldsw)
 {
    extern __shared__ int32_t ldsi[];
+   constexpr bool kViaCC = ssd_via_cc<OP, ACC>();
+   constexpr int TOP = kViaCC ? kCC : OP;   // the op of each tap
    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
    const int x0 = blockIdx.x * (kWaves * R), y0 = blockIdx.y * kTileRows;
    const int cx = tw / 2, cy = th / 2;
@@ -269,19 +279,19 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
          for (; u0 + kIntU <= tw; u0 += kIntU) {
             int32_t t[kIntU], sw[R + kIntU - 1];
 #pragma unroll
-            for (int k = 0; k < kIntU; ++k) t[k] = sad_key<OP, TYPE, ACC>(trow[u0 + k]);
+            for (int k = 0; k < kIntU; ++k) t[k] = sad_key<TOP, TYPE, ACC>(trow[u0 + k]);
 #pragma unroll
-            for (int j = 0; j < R + kIntU - 1; ++j) sw[j] = sad_key<OP, TYPE, ACC>(lrow[u0 + j]);
+            for (int j = 0; j < R + kIntU - 1; ++j) sw[j] = sad_key<TOP, TYPE, ACC>(lrow[u0 + j]);
 #pragma unroll
             for (int k = 0; k < kIntU; ++k)
 #pragma unroll
-               for (int r = 0; r < R; ++r) acc[r] = itap<OP, ACC>(acc[r], sw[r + k], t[k]);
+               for (int r = 0; r < R; ++r) acc[r] = itap<TOP, ACC>(acc[r], sw[r + k], t[k]);
          }
          for (; u0 < tw; ++u0) {
-            const int32_t t = sad_key<OP, TYPE, ACC>(trow[u0]);
+            const int32_t t = sad_key<TOP, TYPE, ACC>(trow[u0]);
 #pragma unroll
             for (int r = 0; r < R; ++r)
-               acc[r] = itap<OP, ACC>(acc[r], sad_key<OP, TYPE, ACC>(lrow[u0 + r]), t);
+               acc[r] = itap<TOP, ACC>(acc[r], sad_key<TOP, TYPE, ACC>(lrow[u0 + r]), t);
          }
       }
    }
@@ -290,7 +300,18 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
    if (oy >= H) return;
    float *drow = reinterpret_cast<float *>(dst + (size_t)oy * dstep);
    float val[R];
-   if (OP == kSSDNorm || OP == kCCNorm) {
+   if (kViaCC) {
+      // exact: every term is an integer below 2^53
+      const double tt = tsums[0];
+      double ss[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) ss[r] = window_sqsum(integ, W, H, x0 + colbase + r, oy, tw, th);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+         const double d = ss[r] - 2.0 * (double)acc[r] + tt;
+         val[r] = OP == kSSDNorm ? (float)(d / sqrt(ss[r] * tt)) : (float)d;
+      }
+   } else if (OP == kSSDNorm || OP == kCCNorm) {
       // exact sums over an FP64 division: all integral corners first
       const double tt = tsums[0];
       double ss[R];
