@@ -233,6 +233,23 @@ template <int OP, int TYPE, class ACC> __device__ __forceinline__ int32_t sad_ke
 
 constexpr int kIntU = 8;   // template columns per unrolled chunk of prox_tile_int
 
+// Four 8-bit values (low bytes of a..d) in one dword, a in the low byte.
+__device__ __forceinline__ uint32_t pack4(int32_t a, int32_t b, int32_t c, int32_t d)
+{
+   const uint32_t lo = __builtin_amdgcn_perm((uint32_t)b, (uint32_t)a, 0x0C0C0400u);
+   const uint32_t hi = __builtin_amdgcn_perm((uint32_t)d, (uint32_t)c, 0x0C0C0400u);
+   return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+}
+
+// Four taps at once on packed bytes: v_dot4_u32_u8 / v_dot4_i32_i8 for the cross term,
+// v_sad_u8 (bytes kept in 0..255 by sad_key) for AbsDistance.
+template <int OP, int TYPE> __device__ __forceinline__ int32_t itap4(int32_t acc, uint32_t s, uint32_t t)
+{
+   if (OP == kSAD) return (int32_t)__builtin_amdgcn_sad_u8(s, t, (uint32_t)acc);
+   if (TYPE == SImage::S8) return __builtin_amdgcn_sdot4((int32_t)s, (int32_t)t, acc, false);
+   return (int32_t)__builtin_amdgcn_udot4(s, t, (uint32_t)acc, false);
+}
+
 // With int32 sums (8-bit sources) SqrDistance and SqrDistance_Norm take the cross term only,
 // one multiply-add per tap, and finish as sum s^2 - 2 sum s*t + sum t^2: the window's sum of
 // squares from the FP64 integral (an exact integer below 2^53), sum t^2 from tsums[0].
@@ -253,6 +270,9 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
    extern __shared__ int32_t ldsi[];
    constexpr bool kViaCC = ssd_via_cc<OP, ACC>();
    constexpr int TOP = kViaCC ? kCC : OP;   // the op of each tap
+   // 8-bit sources with int32 sums: four taps per instruction on packed bytes
+   constexpr bool kPacked = sizeof(ACC) == 4 && (TYPE == SImage::U8 || TYPE == SImage::S8) &&
+                            (TOP == kCC || TOP == kCCNorm || TOP == kSAD);
    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
    const int x0 = blockIdx.x * (kWaves * R), y0 = blockIdx.y * kTileRows;
    const int cx = tw / 2, cy = th / 2;
@@ -276,6 +296,35 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
          // R + kIntU - 1 window values they meet, then R x kIntU taps (exact integer sums,
          // so the order is free); LDS columns read: colbase .. colbase + R + tw - 2
          int u0 = 0;
+         if constexpr (kPacked) {
+            // the window as bytes: aligned dwords q[m] = sw[4m .. 4m+3] (zero past the end),
+            // the dword starting at byte j = 4m + a by v_alignbyte_b32 of q[m+1]:q[m]
+            constexpr int NQ = (R + kIntU - 1 + 3) / 4;
+            for (; u0 + kIntU <= tw; u0 += kIntU) {
+               int32_t sw[4 * NQ];
+#pragma unroll
+               for (int j = 0; j < 4 * NQ; ++j)
+                  sw[j] = j < R + kIntU - 1 ? sad_key<TOP, TYPE, ACC>(lrow[u0 + j]) : 0;
+               uint32_t q[NQ], tq[kIntU / 4];
+#pragma unroll
+               for (int m = 0; m < NQ; ++m) q[m] = pack4(sw[4 * m], sw[4 * m + 1], sw[4 * m + 2], sw[4 * m + 3]);
+#pragma unroll
+               for (int g = 0; g < kIntU / 4; ++g)
+                  tq[g] = pack4(sad_key<TOP, TYPE, ACC>(trow[u0 + 4 * g]),
+                                sad_key<TOP, TYPE, ACC>(trow[u0 + 4 * g + 1]),
+                                sad_key<TOP, TYPE, ACC>(trow[u0 + 4 * g + 2]),
+                                sad_key<TOP, TYPE, ACC>(trow[u0 + 4 * g + 3]));
+#pragma unroll
+               for (int g = 0; g < kIntU / 4; ++g)
+#pragma unroll
+                  for (int r = 0; r < R; ++r) {
+                     const int j = r + 4 * g;
+                     const uint32_t pj = (j & 3) == 0 ? q[j >> 2]
+                                         : __builtin_amdgcn_alignbyte(q[(j >> 2) + 1], q[j >> 2], j & 3);
+                     acc[r] = itap4<TOP, TYPE>(acc[r], pj, tq[g]);
+                  }
+            }
+         }
          for (; u0 + kIntU <= tw; u0 += kIntU) {
             int32_t t[kIntU], sw[R + kIntU - 1];
 #pragma unroll
