@@ -297,14 +297,23 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
          // so the order is free); LDS columns read: colbase .. colbase + R + tw - 2
          int u0 = 0;
          if constexpr (kPacked) {
-            // the window as bytes: aligned dwords q[m] = sw[4m .. 4m+3] (zero past the end),
-            // the dword starting at byte j = 4m + a by v_alignbyte_b32 of q[m+1]:q[m]
+            // the window as bytes: aligned dwords q[m] = sw[4m .. 4m+3], the dword starting
+            // at byte j = 4m + a by v_alignbyte_b32 of q[m+1]:q[m].  sw is read as 16-byte
+            // LDS loads (lrow + u0 is 16-byte aligned: ldsw, colbase and u0 are multiples of
+            // 4); the last of them takes one column past the R + kIntU - 1 used, at most
+            // column ncols, inside the row pitch (ldsw >= ncols + 4), and its value is
+            // never used.
             constexpr int NQ = (R + kIntU - 1 + 3) / 4;
             for (; u0 + kIntU <= tw; u0 += kIntU) {
                int32_t sw[4 * NQ];
+               const int4 *w4 = reinterpret_cast<const int4 *>(lrow + u0);
 #pragma unroll
-               for (int j = 0; j < 4 * NQ; ++j)
-                  sw[j] = j < R + kIntU - 1 ? sad_key<TOP, TYPE, ACC>(lrow[u0 + j]) : 0;
+               for (int m = 0; m < NQ; ++m) {
+                  const int4 v4 = w4[m];
+                  sw[4 * m] = v4.x; sw[4 * m + 1] = v4.y; sw[4 * m + 2] = v4.z; sw[4 * m + 3] = v4.w;
+               }
+#pragma unroll
+               for (int j = 0; j < 4 * NQ; ++j) sw[j] = sad_key<TOP, TYPE, ACC>(sw[j]);
                uint32_t q[NQ], tq[kIntU / 4];
 #pragma unroll
                for (int m = 0; m < NQ; ++m) q[m] = pack4(sw[4 * m], sw[4 * m + 1], sw[4 * m + 2], sw[4 * m + 3]);
