@@ -358,25 +358,30 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
    if (oy >= H) return;
    float *drow = reinterpret_cast<float *>(dst + (size_t)oy * dstep);
    float val[R];
-   if (kViaCC) {
-      // exact: every term is an integer below 2^53
+   if constexpr (kViaCC || OP == kSSDNorm || OP == kCCNorm) {
+      // the sums are integers, exact in FP64 below 2^53 (always, for 8-bit sources).
+      // Four outputs at a time, each group
+      // stored before the next group's integral corners are loaded (the scheduling
+      // barriers keep them apart): all sixteen outputs' corners at once would hold about
+      // 100 more VGPRs and cost a wave per SIMD
       const double tt = tsums[0];
-      double ss[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) ss[r] = window_sqsum(integ, W, H, x0 + colbase + r, oy, tw, th);
+      for (int g = 0; g < R; g += 4) {
+         double ss[4];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-         const double d = ss[r] - 2.0 * (double)acc[r] + tt;
-         val[r] = OP == kSSDNorm ? (float)(d / sqrt(ss[r] * tt)) : (float)d;
+         for (int r = 0; r < 4; ++r)
+            ss[r] = window_sqsum(integ, W, H, x0 + colbase + g + r, oy, tw, th);
+#pragma unroll
+         for (int r = 0; r < 4; ++r) {
+            const double c = (double)acc[g + r];
+            const double d = kViaCC ? ss[r] - 2.0 * c + tt : c;
+            const float v = OP == kSSD ? (float)d : (float)(d / sqrt(ss[r] * tt));
+            const int ox = x0 + colbase + g + r;
+            if (ox < W) drow[ox * nch + ch] = v;
+         }
+         __builtin_amdgcn_sched_barrier(0);
       }
-   } else if (OP == kSSDNorm || OP == kCCNorm) {
-      // exact sums over an FP64 division: all integral corners first
-      const double tt = tsums[0];
-      double ss[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) ss[r] = window_sqsum(integ, W, H, x0 + colbase + r, oy, tw, th);
-#pragma unroll
-      for (int r = 0; r < R; ++r) val[r] = (float)((double)acc[r] / sqrt(ss[r] * tt));
+      return;
    } else {
 #pragma unroll
       for (int r = 0; r < R; ++r) val[r] = (float)acc[r];
