@@ -206,10 +206,10 @@ __device__ __forceinline__ void stage_tile_int(int32_t *lds, int ldsw, int nrows
    }
 }
 
-// One tap.  int32 sums come only from 8-bit sources, whose values and differences fit the
-// 24-bit multiplies (v_mad_i32_i24) and, for SAD, v_sad_u16 (the absolute differences of
-// both 16-bit halves plus c; its operands are kept in 0..255, see sad_key, so the high
-// halves are zero).
+// One tap.  int32 sums come from 8-bit sources, whose values and differences fit the 24-bit
+// multiplies (v_mad_i32_i24), and for SAD from 16-bit sources too; SAD takes v_sad_u16 (the
+// absolute differences of both 16-bit halves plus c; its operands are kept in 0..65535, see
+// sad_key, so the high halves are zero).
 template <int OP, class ACC> __device__ __forceinline__ ACC itap(ACC acc, int32_t s, int32_t t)
 {
    if constexpr (sizeof(ACC) == 4) {
@@ -223,12 +223,13 @@ template <int OP, class ACC> __device__ __forceinline__ ACC itap(ACC acc, int32_
    }
 }
 
-// Operand of itap: for SAD with int32 sums on a signed 8-bit source, the value + 128, which
-// maps -128..127 in order onto 0..255 (a sign-extended negative value would put 0xffff in
-// v_sad_u16's high half); any other value unchanged.
+// Operand of itap: for SAD with int32 sums on a signed source, the value + 128 (8-bit) or
+// + 32768 (16-bit), which maps it in order onto 0..255 or 0..65535 (a sign-extended
+// negative value would put 0xffff in v_sad_u16's high half); any other value unchanged.
 template <int OP, int TYPE, class ACC> __device__ __forceinline__ int32_t sad_key(int32_t v)
 {
-   return (OP == kSAD && sizeof(ACC) == 4 && TYPE == SImage::S8) ? v + 128 : v;
+   if (OP != kSAD || sizeof(ACC) != 4) return v;
+   return TYPE == SImage::S8 ? v + 128 : TYPE == SImage::S16 ? v + 32768 : v;
 }
 
 constexpr int kIntU = 8;   // template columns per unrolled chunk of prox_tile_int
@@ -335,11 +336,19 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
             }
          }
          for (; u0 + kIntU <= tw; u0 += kIntU) {
-            int32_t t[kIntU], sw[R + kIntU - 1];
+            // the same 16-byte window reads as above
+            constexpr int NQ = (R + kIntU - 1 + 3) / 4;
+            int32_t t[kIntU], sw[4 * NQ];
 #pragma unroll
             for (int k = 0; k < kIntU; ++k) t[k] = sad_key<TOP, TYPE, ACC>(trow[u0 + k]);
+            const int4 *w4 = reinterpret_cast<const int4 *>(lrow + u0);
 #pragma unroll
-            for (int j = 0; j < R + kIntU - 1; ++j) sw[j] = sad_key<TOP, TYPE, ACC>(lrow[u0 + j]);
+            for (int m = 0; m < NQ; ++m) {
+               const int4 v4 = w4[m];
+               sw[4 * m] = v4.x; sw[4 * m + 1] = v4.y; sw[4 * m + 2] = v4.z; sw[4 * m + 3] = v4.w;
+            }
+#pragma unroll
+            for (int j = 0; j < R + kIntU - 1; ++j) sw[j] = sad_key<TOP, TYPE, ACC>(sw[j]);
 #pragma unroll
             for (int k = 0; k < kIntU; ++k)
 #pragma unroll
@@ -512,10 +521,13 @@ static void launch_op(hipStream_t s, const View &S, const View &T, void *dst, in
 
    if (small_int) {
       KernelTimer kt(s, "prox_tile_int");
-      // int32 accumulation is exact when taps * max|term| < 2^31 (8-bit sources,
-      // templates up to 33025 taps); 16-bit sources accumulate in int64.
+      // int32 accumulation is exact when taps * max|term| < 2^31: 8-bit sources up to 33025
+      // taps, and AbsDistance on 16-bit sources up to 32768 taps; the other 16-bit cases
+      // accumulate in int64.
       const bool bits8 = S.type == SImage::U8 || S.type == SImage::S8;
-      const bool fits32 = bits8 && (long long)tw * th * 65025LL < (1LL << 31);
+      const long long taps = (long long)tw * th;
+      const bool fits32 = bits8 ? taps * 65025LL < (1LL << 31)
+                                : OP == kSAD && taps * 65535LL < (1LL << 31);
       OCIP_TYPE_SWITCH(S.type, TT, {
          if (fits32) {
             if (R == 16) OCIP_TILE_INT(TT, 16, int32_t); else OCIP_TILE_INT(TT, 8, int32_t);
