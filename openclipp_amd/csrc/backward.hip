@@ -112,18 +112,41 @@ __global__ __launch_bounds__(kThreads) void weight_corr(const char *M, int mstep
                   : 0.f;
    }
    __syncthreads();
-   const int ntaps = tw * th;
-   float *out = partial + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * ntaps;
-   for (int tap = threadIdx.x; tap < ntaps; tap += kThreads) {
-      const int v = tap / tw, u = tap - v * tw;
-      float acc = 0.f;
+   // each thread takes four adjacent taps (u0 .. u0 + 3, v) of one template row, so every
+   // window value read feeds four FMAs; each tap still sums over (py, px) in order. The window
+   // row is read as 16-byte LDS loads (ldsw and u0 are multiples of 4, ldsw = 4 mod 32): they
+   // reach column u0 + TILE + 3 <= tw + TILE + 2 < ldsw, zeros past the staged columns, and
+   // taps past tw are not stored.
+   const int qw = (tw + 3) / 4, ngroups = qw * th;
+   float *out = partial + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * (tw * th);
+   for (int gi = threadIdx.x; gi < ngroups; gi += kThreads) {
+      const int v = gi / qw, u0 = 4 * (gi - v * qw);
+      float acc[4] = {0.f, 0.f, 0.f, 0.f};
       for (int py = 0; py < TILE; ++py) {
-         const float *mrow = mt + py * TILE;
-         const float *srow = st + (size_t)(py + v) * ldsw + u;
-#pragma unroll 16
-         for (int px = 0; px < TILE; ++px) acc = __builtin_fmaf(mrow[px], srow[px], acc);
+         const float4 *m4 = reinterpret_cast<const float4 *>(mt + py * TILE);
+         const float4 *s4 = reinterpret_cast<const float4 *>(st + (size_t)(py + v) * ldsw + u0);
+#pragma unroll
+         for (int c = 0; c < TILE / 16; ++c) {
+            float m[16], w[20];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+               const float4 a = m4[4 * c + k];
+               m[4 * k] = a.x; m[4 * k + 1] = a.y; m[4 * k + 2] = a.z; m[4 * k + 3] = a.w;
+            }
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+               const float4 b = s4[4 * c + k];
+               w[4 * k] = b.x; w[4 * k + 1] = b.y; w[4 * k + 2] = b.z; w[4 * k + 3] = b.w;
+            }
+#pragma unroll
+            for (int px = 0; px < 16; ++px)
+#pragma unroll
+               for (int j = 0; j < 4; ++j) acc[j] = __builtin_fmaf(m[px], w[px + j], acc[j]);
+         }
       }
-      out[tap] = acc;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+         if (u0 + j < tw) out[v * tw + u0 + j] = acc[j];
    }
 }
 
@@ -255,7 +278,10 @@ void launch_backward(hipStream_t s, int mode, const View &S, const View &G, cons
    }
 
    if (dT) {
-      const int ldsw = (tile + tw - 1 + 3) / 4 * 4 + 1;
+      // a multiple of 4 floats = 4 mod 32 (16-byte window loads, rows on different banks),
+      // at least tile + tw + 3
+      int ldsw = (tile + tw - 1 + 4 + 3) / 4 * 4;
+      while (ldsw % 32 != 4) ldsw += 4;
       const size_t lds = (size_t)tile * tile * 4 + (size_t)(tile + th - 1) * ldsw * 4;
       dim3 grid(nbx, nby);
       if (tile == 64) {
