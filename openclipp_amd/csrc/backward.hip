@@ -166,13 +166,24 @@ __global__ __launch_bounds__(kThreads) void grad_template(const float *partial, 
    sc = block_sum_d(sc, red);
    const double a = mode == 0 ? 1.0 : -2.0;
    const double b = mode == 0 ? -sc / tsums[0] : 2.0 * sc;
-   const int ntaps = tw * th;
-   for (int tap = blockIdx.x * kThreads + threadIdx.x; tap < ntaps; tap += gridDim.x * kThreads) {
+   // 64 taps per workgroup (lane = tap); the four waves sum the blocks k = wave (mod 4),
+   // then lane's four sums are added in wave order
+   __shared__ double part[4][64];
+   const int ntaps = tw * th, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   for (int t0 = blockIdx.x * 64; t0 < ntaps; t0 += gridDim.x * 64) {
+      const int tap = t0 + lane;
       double r = 0;
-      for (int k = 0; k < nblocks; ++k) r += partial[(size_t)k * ntaps + tap];
-      const int v = tap / tw, u = tap - v * tw;
-      const double t = tstage[(size_t)v * tpitch + u];
-      reinterpret_cast<float *>(dT + (size_t)v * dtstep)[u] = (float)(a * r + b * t);
+      if (tap < ntaps)
+         for (int k = wave; k < nblocks; k += 4) r += partial[(size_t)k * ntaps + tap];
+      part[wave][lane] = r;
+      __syncthreads();
+      if (wave == 0 && tap < ntaps) {
+         r = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+         const int v = tap / tw, u = tap - v * tw;
+         const double t = tstage[(size_t)v * tpitch + u];
+         reinterpret_cast<float *>(dT + (size_t)v * dtstep)[u] = (float)(a * r + b * t);
+      }
+      __syncthreads();
    }
 }
 
@@ -295,7 +306,7 @@ void launch_backward(hipStream_t s, int mode, const View &S, const View &G, cons
          weight_corr<32><<<grid, kThreads, lds, s>>>(M, mstep, Sp, S.step, W, H, tw, th, partial, ldsw);
       }
       launch_check("weight_corr");
-      grad_template<<<(tw * th + kThreads - 1) / kThreads, kThreads, 0, s>>>(
+      grad_template<<<(tw * th + 63) / 64, kThreads, 0, s>>>(
           partial, nbx * nby, scal, nscal, tstage, tpitch, tsums, tw, th, mode, dT, dtstep);
       launch_check("grad_template");
    }
