@@ -550,11 +550,11 @@ static void launch_op(hipStream_t s, const View &S, const View &T, void *dst, in
 #undef OCIP_TILE_INT
 }
 
-void launch_crosscorr_f32_anchored(hipStream_t s, const View &M, const float *tstage, int tpitch,
+template <int R>
+static void crosscorr_f32_anchored(hipStream_t s, const View &M, const float *tstage, int tpitch,
                                    int tw, int th, int ax, int ay, float *dst, int dstep)
 {
    const int W = M.width, H = M.height;
-   constexpr int R = 16;
    const int ncols = kWaves * R + tw - 1;
    const int ldsw = lds_pitch(ncols);
    const int rows_fit = kLdsBudget / (ldsw * 4) - (kTileRows - 1);
@@ -568,6 +568,19 @@ void launch_crosscorr_f32_anchored(hipStream_t s, const View &M, const float *ts
        static_cast<const char *>(M.ptr), W, H, M.step, 1, 0, tstage, tpitch, tw, th, nullptr,
        nullptr, reinterpret_cast<char *>(dst), dstep, vb, ldsw, ax, ay);
    launch_check("prox_tile_f32");
+}
+
+void launch_crosscorr_f32_anchored(hipStream_t s, const View &M, const float *tstage, int tpitch,
+                                   int tw, int th, int ax, int ay, float *dst, int dstep)
+{
+   // 16 outputs per lane, unless that leaves fewer than two workgroups per CU (images below
+   // about 2048 x 1024): then 8, with twice the workgroups
+   const long long tiles16 = (long long)((M.width + kWaves * 16 - 1) / (kWaves * 16)) *
+                             ((M.height + kTileRows - 1) / kTileRows);
+   if (tiles16 >= 512)
+      crosscorr_f32_anchored<16>(s, M, tstage, tpitch, tw, th, ax, ay, dst, dstep);
+   else
+      crosscorr_f32_anchored<8>(s, M, tstage, tpitch, tw, th, ax, ay, dst, dstep);
 }
 
 void launch_proximity(hipStream_t s, int op, const View &S, const View &T, void *dst, int dstep,
