@@ -295,7 +295,8 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
          const int32_t *trow = tp + (size_t)(v0 + v) * tpitch;
          // the template row in chunks of kIntU columns: kIntU template values and the
          // R + kIntU - 1 window values they meet, then R x kIntU taps (exact integer sums,
-         // so the order is free); LDS columns read: colbase .. colbase + R + tw - 2
+         // so the order is free); LDS columns used: colbase .. colbase + R + tw - 2 (the
+         // 16-byte reads below take up to one more, inside the row pitch)
          int u0 = 0;
          if constexpr (kPacked) {
             // the window as bytes: aligned dwords q[m] = sw[4m .. 4m+3], the dword starting
