@@ -488,13 +488,14 @@ bool rows_aligned(const View &S)
 }
 
 // ---------------------------------------------------------------------------------------
-// Autotuner: the kernel variant for each (kind, precision, KC, mode, image size, template
-// height).  Variants 2 and 4: crosscorr_mfma with NT = 2 or 4 32-column n-tiles per wave
+// Autotuner: the kernel variant for each (kind, precision, mode, template size, image size,
+// row pitch and alignment, device).  Variants 2 and 4: crosscorr_mfma with NT = 2 or 4 32-column n-tiles per wave
 // (64- or 128-column output tiles); 14 and 16: crosscorr_mfma_pair with NT = 4 or 6 16-column
-// n-tiles (64- or 96-column tiles); 24: crosscorr_mfma_strip (NT = 4).  The first call with a new key times every candidate
-// with HIP events on the call's own inputs and keeps the fastest; the table is kept for the
-// process and, when the environment variable OCIP_AUTOTUNE_CACHE names a file, read from
-// and appended to that file.
+// n-tiles (64- or 96-column tiles); 24: crosscorr_mfma_strip (NT = 4).  The first call with a
+// new key times every candidate with HIP events on the call's own inputs and keeps the
+// fastest; the table is kept for the process and, when the environment variable
+// OCIP_AUTOTUNE_CACHE names a file, read from and appended to that file.  An entry is used
+// only when it names one of the calling op's candidates (a variant it can run).
 bool known_variant(int v) { return v == 2 || v == 4 || v == 14 || v == 16 || v == 24; }
 
 std::map<std::string, int> &tune_table()
@@ -521,7 +522,13 @@ int tuned_variant(const std::string &key, hipStream_t s, const std::function<voi
 {
    auto &table = tune_table();
    auto it = table.find(key);
-   if (it != table.end()) return it->second;
+   // a table or cache-file entry is used only when it names one of this call's candidates,
+   // i.e. a variant whose preconditions this call meets; anything else is tuned again
+   if (it != table.end()) {
+      for (int c : candidates)
+         if (c == it->second) return c;
+      table.erase(it);
+   }
    hipEvent_t e0, e1;
    hip_check(hipEventCreate(&e0), "event");
    hip_check(hipEventCreate(&e1), "event");
@@ -556,10 +563,33 @@ int tuned_variant(const std::string &key, hipStream_t s, const std::function<voi
    return best;
 }
 
-std::string tune_key(const char *kind, int precision, int KC, int mode, int W, int H, int th)
+// The device the tuning was measured on: architecture name and CU count of the current GPU.
+const std::string &device_tag()
 {
-   char buf[160];
-   snprintf(buf, sizeof buf, "%s:p%d:kc%d:m%d:%dx%d:th%d", kind, precision, KC, mode, W, H, th);
+   static std::string tag;
+   if (tag.empty()) {
+      int dev = 0;
+      hip_check(hipGetDevice(&dev), "device");
+      hipDeviceProp_t prop;
+      hip_check(hipGetDeviceProperties(&prop, dev), "device");
+      char buf[96];
+      snprintf(buf, sizeof buf, "%s:cu%d", prop.gcnArchName, prop.multiProcessorCount);
+      for (char *c = buf; *c; ++c)
+         if (*c == ' ') *c = '_';  // one whitespace-free token per key in the cache file
+      tag = buf;
+   }
+   return tag;
+}
+
+// Everything a variant's validity and speed depend on: the templates (`shape`: a single
+// template's width and height, from which the fragment counts KC, KC2 and the width blocks
+// follow, or a batch bucket's KC and tallest template), the image size, row pitch (the strip
+// kernel's 32-bit offsets) and 16-byte alignment, and the device.
+std::string tune_key(const char *kind, int precision, int mode, const char *shape, const View &S)
+{
+   char buf[256];
+   snprintf(buf, sizeof buf, "%s:p%d:m%d:%s:%dx%d:s%d:a%d:%s", kind, precision, mode, shape,
+            S.width, S.height, S.step, rows_aligned(S) ? 1 : 0, device_tag().c_str());
    return buf;
 }
 
@@ -654,15 +684,29 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
                          (long long)S.height * S.step <= (long long)mfma::kRingOob &&
                          (long long)S.height * dstep <= (long long)mfma::kRingOob;
    // the contraction is the same for CrossCorr and CrossCorr_Norm: one tuning key
-   const std::string key = tune_key("single", precision, KC, mfma::kModeCC, S.width, S.height, th);
+   char shape[48];
+   snprintf(shape, sizeof shape, "tw%d:th%d", tw, th);
+   const std::string key = tune_key("single", precision, mfma::kModeCC, shape, S);
    // OCIP_MFMA_VARIANT (2, 4, 14, 16 or 24) names a variant and skips the autotuner
    int forced = 0;
    if (const char *e = getenv("OCIP_MFMA_VARIANT")) {
       const int v = atoi(e);
       if (known_variant(v) && (pair_ok || v < 10) && (strip_ok || v != 24)) forced = v;
    }
-   const auto known = tune_table().find(key);
-   const int chosen = forced ? forced : known == tune_table().end() ? 0 : known->second;
+   // the variants whose preconditions this call meets, in the autotuner's order
+   const std::initializer_list<int> cands_strip = {24, 16, 14, 4, 2}, cands_pair = {16, 14, 4, 2},
+                                    cands_plain = {4, 2};
+   const std::initializer_list<int> &cands = strip_ok ? cands_strip : pair_ok ? cands_pair
+                                                                              : cands_plain;
+   // a tuned variant that this call cannot run counts as untuned (0: prepare every candidate's
+   // fragments, then tune again)
+   int chosen = forced;
+   if (!chosen) {
+      const auto known = tune_table().find(key);
+      if (known != tune_table().end())
+         for (int c : cands)
+            if (c == known->second) chosen = c;
+   }
 
    char *wk = static_cast<char *>(work);
    const size_t nfrag = (size_t)nub * th * KC;
@@ -737,10 +781,7 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
                     (S.height + mfma::kTileRows - 1) / mfma::kTileRows);
       launch_single_p(precision, s, L);
    };
-   const int variant = forced ? forced
-                       : strip_ok ? tuned_variant(key, s, run, {24, 16, 14, 4, 2})
-                       : pair_ok ? tuned_variant(key, s, run, {16, 14, 4, 2})
-                                 : tuned_variant(key, s, run, {4, 2});
+   const int variant = chosen ? chosen : tuned_variant(key, s, run, cands);
    {
       KernelTimer kt(s, "crosscorr_mfma");
       run(variant);
@@ -907,8 +948,9 @@ void run_batch_mfma(hipStream_t s, BatchCache *cache, int num_cus, int precision
                reset();
                bucket_launch(b, NT);
             };
-            nts.push_back(tuned_variant(tune_key("batch", precision, bl[b].KC, mode, S.width, S.height,
-                                            bl[b].th_max), s, run, {4, 2}));
+            char shape[48];
+            snprintf(shape, sizeof shape, "kc%d:thmax%d", bl[b].KC, bl[b].th_max);
+            nts.push_back(tuned_variant(tune_key("batch", precision, mode, shape, S), s, run, {4, 2}));
          }
       }
       // record the whole batch once

@@ -211,7 +211,6 @@ void launch_backward(hipStream_t s, int mode, const View &S, const View &G, cons
                      int tpitch, int tw, int th, const double *tsums, const double *integ_ss,
                      void *work, void *aux, char *dS, int dsstep, char *dT, int dtstep);
 
-// Min / max with first-index (row-major) tie break.
 // Proximity through 2-D FFTs (FP64): op kSSD, kSSDNorm, kCC or kCCNorm on an F32 1-channel
 // source and template into an F32 dest.  integ: the exclusive FP64 sum-of-squares integral
 // (all ops but kCC); tsums[0] = sum t^2.  fft_sizes gives the padded transform size or
@@ -232,6 +231,8 @@ void launch_fft(hipStream_t s, FftState *st, bool forward, const View &real, con
 // blocking.  partials: sum_partials_bytes of device scratch.
 size_t sum_partials_bytes(int width, int height);
 void launch_sum(hipStream_t s, bool sqr, const View &src, void *partials, double *result);
+// Min / max of a 1-channel image and the pixel holding it, in the reference's reduction order
+// (its tie-breaks and float partials), blocking.
 size_t minmax_partials_bytes(int width, int height);
 void launch_minmax(hipStream_t s, bool want_max, const View &src, void *partials,
                    double *host_result, int *ix, int *iy);

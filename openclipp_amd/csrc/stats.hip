@@ -2,12 +2,21 @@
 // sum and sum of squares.
 //
 // Reference: Statistics::Min / Max (C++/programs/Statistics.cpp:244-270) with the
-// min_coord / max_coord kernels (cl-files/Statistics.cl:342-517) and the host final reduce
-// (C++/programs/StatisticsHelpers.cpp:126-145).  The reference keeps partial results as
-// float and breaks ties by the order of its reduction tree; here values are compared in
-// FP64 (exact for every element type) and ties go to the first pixel in row-major order.
-// NaN pixels are skipped.  The call blocks until the result is on the host, as the
-// reference's does.
+// min_coord / max_coord kernels (cl-files/Statistics.cl:332-425, _flush variant :427-507) and
+// the host final reduce (C++/programs/StatisticsHelpers.cpp:105-145).  Same pixels per
+// work-item, same comparisons, same pairings and the same float partials, so ties and NaNs
+// resolve to the reference's pixel:
+//   * work-group (256 threads) = 256 columns x 16 rows; thread lid = 16 ly + lx scans row
+//     16 gyg + ly at columns 256 gxg + lx + 16 i, i = 0..15, keeping the first of equal values;
+//   * halving tree over lid (k = 128 .. 1): position lid takes lid + k only when both hold
+//     pixels and lid + k is strictly better (k = 128, 64 through LDS, k <= 32 in wave 0 by
+//     shuffles: positions < k only ever read positions < 2k, written at the level before);
+//   * each group's extreme is kept as float with its pixel; the final pass takes, as the
+//     reference's host loop does, the lowest group id among the best floats (group 0 if its
+//     float is NaN, since the host loop never replaces a NaN it starts from).
+// Values are compared as doubles (every element type converts exactly, so the comparisons are
+// those of the element type) and the result is the float partial, as the reference returns.
+// The call blocks until the result is on the host, as the reference's does.
 //
 // Sum / SumSqr: reference Statistics::Sum / SumSqr (C++/programs/Statistics.cpp:350-370) with
 // reduce_sum / reduce_sum_sqr (cl-files/Statistics.cl:47-121, 279-288, 310-311) and the host
@@ -24,70 +33,98 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kMaxBlocks = 1024;
 
-struct Cand {
-   double v;
-   long long i;   // linear index y*W + x; -1 = none
+// One group's (or the final) result: the extreme as float and its pixel.
+struct MinMaxPart {
+   float v;
+   int x, y, pad;
 };
 
-template <bool MAX> __device__ __forceinline__ bool better(const Cand &a, const Cand &b)
+template <bool MAX> __device__ __forceinline__ bool strictly_better(double b, double a)
 {
-   if (a.i < 0) return false;
-   if (b.i < 0) return true;
-   if (MAX ? a.v > b.v : a.v < b.v) return true;
-   if (a.v == b.v) return a.i < b.i;
-   return false;
-}
-
-template <bool MAX> __device__ Cand block_best(Cand c)
-{
-   __shared__ double sv[kThreads / 64];
-   __shared__ long long si[kThreads / 64];
-#pragma unroll
-   for (int d = 32; d >= 1; d >>= 1) {
-      Cand o{__shfl_xor(c.v, d, 64), __shfl_xor(c.i, d, 64)};
-      if (better<MAX>(o, c)) c = o;
-   }
-   const int wave = threadIdx.x >> 6;
-   if ((threadIdx.x & 63) == 0) { sv[wave] = c.v; si[wave] = c.i; }
-   __syncthreads();
-   if (threadIdx.x == 0) {
-      for (int w = 1; w < kThreads / 64; ++w) {
-         Cand o{sv[w], si[w]};
-         if (better<MAX>(o, c)) c = o;
-      }
-   }
-   return c;
+   return MAX ? b > a : b < a;
 }
 
 template <int TYPE, bool MAX>
-__global__ __launch_bounds__(kThreads) void minmax_partial(const char *src, int W, int H,
-                                                            int step, double *pv,
-                                                            long long *pi)
+__global__ __launch_bounds__(256) void minmax_groups(const char *src, int W, int H, int step,
+                                                      int ngx, MinMaxPart *part)
 {
-   Cand best{0.0, -1};
-   const long long n = (long long)W * H;
-   for (long long k = (long long)blockIdx.x * kThreads + threadIdx.x; k < n;
-        k += (long long)gridDim.x * kThreads) {
-      const int y = (int)(k / W), x = (int)(k - (long long)y * W);
-      double v = (double)ld<TYPE>(src + (size_t)y * step, x);
-      if (v != v) continue;
-      Cand c{v, k};
-      if (better<MAX>(c, best)) best = c;
+   __shared__ double sv[256];
+   __shared__ int sx[256], sy[256], sok[256];
+   const int lid = threadIdx.x, lx = lid & 15, ly = lid >> 4;
+   const int gxg = blockIdx.x % ngx, gyg = blockIdx.x / ngx;
+   const int gx = 256 * gxg + lx, gy = 16 * gyg + ly;
+   int ok = gx < W && gy < H;
+   double v = 0.0;
+   int cx = gx, cy = gy;
+   if (ok) {
+      const char *row = src + (size_t)gy * step;
+      v = (double)ld<TYPE>(row, gx);
+#pragma unroll
+      for (int i = 16; i < 256; i += 16)
+         if (gx + i < W) {
+            const double px = (double)ld<TYPE>(row, gx + i);
+            if (strictly_better<MAX>(px, v)) { v = px; cx = gx + i; }
+         }
    }
-   best = block_best<MAX>(best);
-   if (threadIdx.x == 0) { pv[blockIdx.x] = best.v; pi[blockIdx.x] = best.i; }
+   sv[lid] = v; sx[lid] = cx; sy[lid] = cy; sok[lid] = ok;
+   __syncthreads();
+#pragma unroll
+   for (int k = 128; k >= 64; k >>= 1) {
+      if (lid < k && sok[lid] && sok[lid + k] && strictly_better<MAX>(sv[lid + k], sv[lid])) {
+         sv[lid] = sv[lid + k]; sx[lid] = sx[lid + k]; sy[lid] = sy[lid + k];
+      }
+      __syncthreads();
+   }
+   if (lid < 64) {
+      v = sv[lid]; cx = sx[lid]; cy = sy[lid]; ok = sok[lid];
+#pragma unroll
+      for (int k = 32; k >= 1; k >>= 1) {
+         const double ov = __shfl_down(v, k, 64);
+         const int ox = __shfl_down(cx, k, 64), oy = __shfl_down(cy, k, 64);
+         const int ook = __shfl_down(ok, k, 64);
+         if (lid < k && ok && ook && strictly_better<MAX>(ov, v)) { v = ov; cx = ox; cy = oy; }
+      }
+      if (lid == 0) part[blockIdx.x] = MinMaxPart{(float)v, cx, cy, 0};
+   }
 }
 
+// First group holding the best float (NaN never taken), or group 0 when its float is NaN:
+// the reference's serial host loop.  Each thread scans a strided subset keeping its first
+// best; subsets merge by (value, lower group id), which is order-independent.
 template <bool MAX>
-__global__ __launch_bounds__(kThreads) void minmax_final(int nparts, double *pv, long long *pi)
+__global__ __launch_bounds__(1024) void minmax_final(const MinMaxPart *part, int n,
+                                                     MinMaxPart *out)
 {
-   Cand best{0.0, -1};
-   for (int k = threadIdx.x; k < nparts; k += kThreads) {
-      Cand c{pv[k], pi[k]};
-      if (better<MAX>(c, best)) best = c;
+   __shared__ float rv[1024 / 64];
+   __shared__ int rg[1024 / 64];
+   float v = 0.0f;
+   int g = 0x7fffffff;  // no candidate
+   for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      const float pv = part[k].v;
+      if (pv != pv) continue;
+      if (g == 0x7fffffff || (MAX ? pv > v : pv < v)) { v = pv; g = k; }
    }
-   best = block_best<MAX>(best);
-   if (threadIdx.x == 0) { pv[nparts] = best.v; pi[nparts] = best.i; }
+#pragma unroll
+   for (int d = 32; d >= 1; d >>= 1) {
+      const float ov = __shfl_xor(v, d, 64);
+      const int og = __shfl_xor(g, d, 64);
+      if (og != 0x7fffffff &&
+          (g == 0x7fffffff || (MAX ? ov > v : ov < v) || (ov == v && og < g))) { v = ov; g = og; }
+   }
+   const int wave = threadIdx.x >> 6;
+   if ((threadIdx.x & 63) == 0) { rv[wave] = v; rg[wave] = g; }
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+         const float ov = rv[w];
+         const int og = rg[w];
+         if (og != 0x7fffffff &&
+             (g == 0x7fffffff || (MAX ? ov > v : ov < v) || (ov == v && og < g))) { v = ov; g = og; }
+      }
+      const float p0 = part[0].v;
+      if (p0 != p0 || g == 0x7fffffff) g = 0;
+      *out = part[g];
+   }
 }
 
 int nblocks(int W, int H)
@@ -204,42 +241,35 @@ void launch_sum(hipStream_t s, bool sqr, const View &v, void *partials, double *
    hip_check(hipStreamSynchronize(s), "sync");
 }
 
+static int minmax_groups_count(int W, int H) { return ((W + 255) / 256) * ((H + 15) / 16); }
+
 size_t minmax_partials_bytes(int W, int H)
 {
-   return (size_t)(nblocks(W, H) + 1) * (sizeof(double) + sizeof(long long));
+   return (size_t)(minmax_groups_count(W, H) + 1) * sizeof(MinMaxPart);
 }
 
 void launch_minmax(hipStream_t s, bool want_max, const View &v, void *partials, double *result,
                    int *ix, int *iy)
 {
-   const int nb = nblocks(v.width, v.height);
-   double *pv = static_cast<double *>(partials);
-   long long *pi = reinterpret_cast<long long *>(pv + nb + 1);
+   const int ngx = (v.width + 255) / 256;
+   const int n = minmax_groups_count(v.width, v.height);
+   MinMaxPart *part = static_cast<MinMaxPart *>(partials);
    const char *src = static_cast<const char *>(v.ptr);
    KernelTimer kt(s, "minmax");
    OCIP_TYPE_SWITCH(v.type, TT, {
-      if (want_max) minmax_partial<TT, true><<<nb, kThreads, 0, s>>>(src, v.width, v.height, v.step, pv, pi);
-      else minmax_partial<TT, false><<<nb, kThreads, 0, s>>>(src, v.width, v.height, v.step, pv, pi);
+      if (want_max) minmax_groups<TT, true><<<n, 256, 0, s>>>(src, v.width, v.height, v.step, ngx, part);
+      else minmax_groups<TT, false><<<n, 256, 0, s>>>(src, v.width, v.height, v.step, ngx, part);
    });
-   launch_check("minmax_partial");
-   if (want_max) minmax_final<true><<<1, kThreads, 0, s>>>(nb, pv, pi);
-   else minmax_final<false><<<1, kThreads, 0, s>>>(nb, pv, pi);
+   launch_check("minmax_groups");
+   if (want_max) minmax_final<true><<<1, 1024, 0, s>>>(part, n, part + n);
+   else minmax_final<false><<<1, 1024, 0, s>>>(part, n, part + n);
    launch_check("minmax_final");
-   double hv = 0;
-   long long hi = -1;
-   hip_check(hipMemcpyAsync(&hv, pv + nb, sizeof(double), hipMemcpyDeviceToHost, s), "read");
-   hip_check(hipMemcpyAsync(&hi, pi + nb, sizeof(long long), hipMemcpyDeviceToHost, s), "read");
+   MinMaxPart h{};
+   hip_check(hipMemcpyAsync(&h, part + n, sizeof h, hipMemcpyDeviceToHost, s), "read");
    hip_check(hipStreamSynchronize(s), "sync");
-   if (hi < 0) {
-      // every pixel is NaN
-      *result = __builtin_nan("");
-      *ix = 0;
-      *iy = 0;
-      return;
-   }
-   *result = hv;
-   *iy = (int)(hi / v.width);
-   *ix = (int)(hi - (long long)(*iy) * v.width);
+   *result = (double)h.v;
+   *ix = h.x;
+   *iy = h.y;
 }
 
 }  // namespace ocip

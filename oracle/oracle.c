@@ -14,7 +14,8 @@
  *   orc_integral_exact  exact inclusive 2-D prefix (int128 / long double)
  *   orc_area_sum      4-corner window sum over an inclusive integral
  *                     (cl-files/ImageProximityFFT.cl:55-84)
- *   orc_min_max_indx  min/max value and first pixel holding it
+ *   orc_min_max_indx  min/max value and its pixel in the reference's reduction order
+ *                     (cl-files/Statistics.cl:332-466, StatisticsHelpers.cpp:105-145)
  *
  * Pinning (what checks these functions against something other than themselves)
  * lives in tests/test_oracle.py: scipy.signal.correlate2d, numpy sliding windows,
@@ -445,28 +446,78 @@ int orc_area_sum(const double *I, int W, int H, int w, int h, double *dst)
 }
 
 /* ------------------------------------------------------------------------------------------
- * orc_min_max_indx: min (want_max = 0) or max of a 1-channel image and the first pixel in
- * row-major order that holds it (NaN pixels are skipped).
+ * orc_min_max_indx: Statistics::Min / Max with coordinates (want_max = 0 / 1) of a 1-channel
+ * image, in the reference's order (its tie-breaks and its float partials).
+ *
+ *   cl-files/Statistics.cl:344-425 (REDUCE_POS_STD min_coord / max_coord; the _flush variant
+ *   :436-507 visits the same pixels in the same order on images whose width is a multiple of
+ *   256 and height a multiple of 16):
+ *     work-groups of 16x16 work-items; work-item (lx, ly) of group (gxg, gyg) reads row
+ *     gy = 16 gyg + ly at columns gx + 16 i, gx = 256 gxg + lx, i = 0..15 (:351-354); it is
+ *     active when gx < W and gy < H (:357); it starts from the pixel at i = 0 and takes a later
+ *     one only when strictly smaller (min) / larger (max), comparing in the source's own
+ *     element type (type = SCALAR, :514-515; :359-370);
+ *     the 256 results (lid = 16 ly + lx) are reduced by a halving tree, position lid taking
+ *     position lid + k (k = 128, 64, .., 1) only when both are active and lid + k is strictly
+ *     better (DO_REDUCE, :332-340; :379-417);
+ *     position 0 is stored as float (global float *result, :346) with its coordinates,
+ *     gid = gyg * ngx + gxg (:418-423).
+ *   C++/programs/StatisticsHelpers.cpp:105-145 (ReduceMin / ReduceMax): start from group 0,
+ *     take group g only when its float is strictly better; the value returned is that float.
+ *   Group counts: C++/programs/WorkGroup.h:87-156 with 16 pixels per work-item along x
+ *     (StatisticsHelpers.h:41): ngx = ceil(W / 256), ngy = ceil(H / 16).
+ * NaN follows from the same comparisons (a NaN is never taken, and a NaN already held is
+ * never replaced).  Returns 0.
  * ------------------------------------------------------------------------------------------ */
+typedef struct { double v; int x, y, ok; } mm_item;
+
+/* "b strictly better than a" in the element type: every element type converts to double
+ * exactly, so comparing the doubles is comparing the elements */
+static inline int mm_better(int want_max, double b, double a)
+{
+   return want_max ? b > a : b < a;
+}
+
 int orc_min_max_indx(int want_max, int type, const void *src, int W, int H, int sstep,
                      double *value, int *ix, int *iy)
 {
    if (type < 0 || type > T_F64 || W <= 0 || H <= 0) return -1;
-   int found = 0;
-   double best = 0;
+   const int ngx = (W + 255) / 256, ngy = (H + 15) / 16;
+   float best = 0.0f;
    int bx = 0, by = 0;
-   for (int y = 0; y < H; ++y) {
-      const unsigned char *srow = (const unsigned char *)src + (size_t)y * sstep;
-      for (int x = 0; x < W; ++x) {
-         double v = elem_d(srow, type, x);
-         if (v != v) continue;
-         if (!found || (want_max ? v > best : v < best)) {
-            best = v; bx = x; by = y; found = 1;
+   mm_item it[256];
+   for (int gyg = 0; gyg < ngy; ++gyg)
+      for (int gxg = 0; gxg < ngx; ++gxg) {
+         for (int lid = 0; lid < 256; ++lid) {
+            const int lx = lid & 15, ly = lid >> 4;
+            const int gx = 256 * gxg + lx, gy = 16 * gyg + ly;
+            mm_item *m = &it[lid];
+            m->ok = gx < W && gy < H;
+            if (!m->ok) continue;
+            const unsigned char *srow = (const unsigned char *)src + (size_t)gy * sstep;
+            m->v = elem_d(srow, type, gx);
+            m->x = gx;
+            m->y = gy;
+            for (int i = 16; i < 256; i += 16)
+               if (gx + i < W) {
+                  const double px = elem_d(srow, type, gx + i);
+                  if (mm_better(want_max, px, m->v)) { m->v = px; m->x = gx + i; }
+               }
+         }
+         for (int k = 128; k >= 1; k >>= 1)
+            for (int lid = 0; lid < k; ++lid)
+               if (it[lid].ok && it[lid + k].ok && mm_better(want_max, it[lid + k].v, it[lid].v)) {
+                  it[lid].v = it[lid + k].v;
+                  it[lid].x = it[lid + k].x;
+                  it[lid].y = it[lid + k].y;
+               }
+         const float part = (float)it[0].v;  /* RNE, as OpenCL's implicit conversion */
+         if ((gxg == 0 && gyg == 0) || (want_max ? part > best : part < best)) {
+            best = part; bx = it[0].x; by = it[0].y;
          }
       }
-   }
    *value = best; *ix = bx; *iy = by;
-   return found ? 0 : 1;
+   return 0;
 }
 
 int orc_type_size(int t) { return type_size(t); }

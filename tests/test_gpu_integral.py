@@ -2,8 +2,13 @@
 error paths, through the C ABI.
 
 Integral bar: 8/16-bit integer sources are bit-exact against the exact sums (F64 dest) and
-against the exact sums rounded once to float (F32 dest); float sources are within 1e-13
-relative of the exact (long double) sums for F64 dest and within one float ulp for F32.
+against the exact sums rounded once to float (F32 dest); F32 sources are within 1e-13
+relative of the exact (long double) sums for F64 dest and within one float ulp for F32;
+32-bit integer sources are bit-exact while their sums stay below 2^53, and they (squared) and
+F64 sources are otherwise within 257 * 2^-53 * sum|terms| (one more float rounding for F32).
+
+MinIndx / MaxIndx: the value and pixel of the reference-order oracle (its tie order, its float
+partials), on every element type.
 """
 import ctypes
 
@@ -40,7 +45,7 @@ SHAPES = [(1, 1), (5, 7), (8, 64), (37, 1023), (130, 1025), (64, 4096), (300, 20
 
 
 @pytest.mark.parametrize("shape", SHAPES)
-@pytest.mark.parametrize("dtype", [np.uint8, np.int16])
+@pytest.mark.parametrize("dtype", [np.uint8, np.int8, np.uint16, np.int16])
 @pytest.mark.parametrize("sqr", [False, True])
 def test_integral_integer_exact(ocip, shape, dtype, sqr):
     rng = np.random.default_rng(shape[0] * 7 + shape[1])
@@ -61,6 +66,41 @@ def test_integral_float(ocip, shape, sqr):
     np.testing.assert_allclose(run_integral(ocip, src, sqr, np.float64), exact, rtol=1e-13)
     got32 = run_integral(ocip, src, sqr, np.float32).astype(np.float64)
     assert np.all(np.abs(got32 - exact) <= np.spacing(exact.astype(np.float32)).astype(np.float64))
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (37, 1023), (300, 2049)])
+@pytest.mark.parametrize("dtype", [np.uint32, np.int32, np.float64])
+@pytest.mark.parametrize("sqr", [False, True])
+def test_integral_wide_sources(ocip, shape, dtype, sqr):
+    """32-bit integer and F64 sources over their full range.  The FP64 sums are exact while
+    every partial stays below 2^53 (32-bit sums here); otherwise (squares of 32-bit values,
+    F64 sources) the error of any summation tree of depth d over the terms is at most
+    d 2^-53 sum|terms| (+ 2^-53 per rounded square): d < 256 covers the in-tile column run
+    (32 rows), the wavefront scan and the carries over tile rows and columns.  F32 dest adds
+    one rounding to float."""
+    rng = np.random.default_rng(shape[0] * 5 + shape[1] + np.dtype(dtype).itemsize)
+    if dtype == np.float64:
+        src = rng.standard_normal(shape) * 1e3
+    else:
+        info = np.iinfo(dtype)
+        src = rng.integers(info.min, info.max, size=shape, endpoint=True, dtype=np.int64)
+        src = src.astype(dtype)
+    say("integral", shape, np.dtype(dtype).name, sqr)
+    exact = orc.integral_exact(src, sqr)
+    a = np.abs(src.astype(np.float64))
+    absint = orc.integral_exact(a, sqr)                            # sum |terms|
+    exact_fp64 = dtype != np.float64 and not sqr and absint.max() < 2.0 ** 53
+    got = run_integral(ocip, src, sqr, np.float64)
+    if exact_fp64:
+        np.testing.assert_array_equal(got, exact)
+    else:
+        assert np.all(np.abs(got - exact) <= 257 * 2.0 ** -53 * absint)
+    got32 = run_integral(ocip, src, sqr, np.float32).astype(np.float64)
+    if exact_fp64:
+        np.testing.assert_array_equal(got32, exact.astype(np.float32).astype(np.float64))
+    else:
+        ulp = np.spacing(np.abs(exact).astype(np.float32)).astype(np.float64)
+        assert np.all(np.abs(got32 - exact) <= ulp + 257 * 2.0 ** -53 * absint)
 
 
 def test_integral_4096_u8_sqr_exact(ocip):
@@ -89,19 +129,92 @@ def test_integral_argument_errors(ocip):
 
 
 # ----------------------------------------------------------------------------- statistics
-@pytest.mark.parametrize("dtype", [np.float32, np.uint8, np.int16, np.float64])
-def test_min_max_indx(ocip, dtype):
-    rng = np.random.default_rng(9)
-    if np.issubdtype(dtype, np.integer):
-        src = rng.integers(0, 100, size=(300, 517)).astype(dtype)
-    else:
-        src = rng.random((300, 517)).astype(dtype)
+def _min_max(ocip, src, want_max):
     S = ocip.Image(src)
     v, x, y = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+    fn = ocip.lib.ocipMaxIndx if want_max else ocip.lib.ocipMinIndx
+    ocip.check(fn(None, S.handle, ctypes.byref(v), ctypes.byref(x), ctypes.byref(y)))
+    S.release()
+    return v.value, x.value, y.value
+
+
+def _tie_heavy(rng, dtype, shape):
+    """Few distinct values, so most extremes are tied; 32-bit and F64 values that differ
+    below float precision also tie between the groups' float partials."""
+    if dtype in (np.uint32, np.int32):
+        return (2 ** 30 + rng.integers(0, 4, shape) * 16).astype(dtype)
+    if dtype == np.float64:
+        return 1.0 + rng.integers(0, 3, shape) * 2.0 ** -40
+    if dtype == np.float32:
+        return rng.integers(0, 3, shape).astype(np.float32) * np.float32(0.25)
+    info = np.iinfo(dtype)
+    return rng.integers(max(info.min, -2), min(info.max, 3), shape).astype(dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int8, np.uint16, np.int16, np.uint32,
+                                   np.int32, np.float32, np.float64])
+def test_min_max_indx(ocip, dtype):
+    """Value and pixel equal the reference-order oracle: random values, tie-heavy values on
+    ragged sizes (partial work-groups on both edges), and a single-pixel image."""
+    rng = np.random.default_rng(9)
+    cases = [rng.integers(0, 100, size=(300, 517)).astype(dtype)
+             if np.issubdtype(dtype, np.integer) else rng.random((300, 517)).astype(dtype)]
+    for shape in [(1, 1), (16, 256), (17, 257), (45, 530), (300, 517)]:
+        cases.append(_tie_heavy(rng, dtype, shape))
+    for src in cases:
+        say("min/max", np.dtype(dtype).name, src.shape)
+        for want_max in (True, False):
+            assert _min_max(ocip, src, want_max) == orc.min_max_indx(src, want_max), \
+                (src.shape, want_max)
+
+
+def test_min_max_indx_reference_ties(ocip):
+    """Hand-derived known answers of the reference's order: equal maxima at (1, 0) and
+    (16, 0) give (16, 0) (work-item 0 holds column 16); (5, 1) and (6, 0) give (6, 0)."""
+    img = np.zeros((4, 40), np.uint8)
+    img[0, 1] = img[0, 16] = 9
+    assert _min_max(ocip, img, True) == (9.0, 16, 0)
+    img = np.full((4, 40), 9, np.uint8)
+    img[1, 5] = img[0, 6] = 0
+    assert _min_max(ocip, img, False) == (0.0, 6, 0)
+
+
+def test_min_max_indx_nan(ocip):
+    a = np.ones((20, 300), np.float32)
+    a[3, 7] = 5.0
+    a[1, 1] = np.nan
+    assert _min_max(ocip, a, True) == (5.0, 7, 3)
+    a[0, 0] = np.nan
+    v, x, y = _min_max(ocip, a, True)
+    assert np.isnan(v) and (x, y) == (0, 0)
+
+
+def test_max_indx_of_periodic_ncc_map(ocip):
+    """A saturated match map: the U8 image repeats the template, so CrossCorr_Norm (exact
+    integer sums, exact FP64 window sums) holds bit-identical maxima at every interior
+    period.  The located pixel is the reference's tie choice on that map, on the period
+    grid."""
+    rng = np.random.default_rng(21)
+    tpl = rng.integers(1, 256, (12, 10)).astype(np.uint8)
+    src = np.tile(tpl, (30, 60))                                 # 360 x 600
+    S, T = ocip.Image(src), ocip.Image(tpl)
+    D = ocip.Image(shape=src.shape, dtype=np.float32)
+    ocip.check(ocip.proximity("CrossCorr_Norm", S, T, D), "CrossCorr_Norm")
+    ncc = D.read().copy()
+    for i in (S, T, D):
+        i.release()
+    want = orc.min_max_indx(ncc, True)
+    assert (ncc == np.float32(want[0])).sum() > 100              # really tie-heavy
+    assert _min_max(ocip, ncc, True) == want
+    assert want[1] % 10 == 5 and want[2] % 12 == 6
+
+
+def test_min_max_indx_4096_u8(ocip):
+    """The 4096x4096 U8 workload image: 4096 work-groups, ties everywhere at 0 and 255."""
+    from openclipp_amd import workloads
+    src = workloads.integer_4096()[0]
     for want_max in (True, False):
-        fn = ocip.lib.ocipMaxIndx if want_max else ocip.lib.ocipMinIndx
-        ocip.check(fn(None, S.handle, ctypes.byref(v), ctypes.byref(x), ctypes.byref(y)))
-        assert (v.value, x.value, y.value) == orc.min_max_indx(src, want_max)
+        assert _min_max(ocip, src, want_max) == orc.min_max_indx(src, want_max)
 
 
 SUM_CASES = [((1, 1), 1), ((300, 517), 1), ((64, 4096), 1), ((37, 129), 2), ((20, 300), 4)]

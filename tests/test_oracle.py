@@ -216,18 +216,102 @@ def test_area_sum_matches_window_brute_force():
         np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-10)
 
 
-def test_min_max_indx_first_occurrence():
+def _bitrev8(v):
+    r = np.zeros_like(v)
+    for b in range(8):
+        r |= ((v >> b) & 1) << (7 - b)
+    return r
+
+
+def _min_max_indx_closed_form(src, want_max):
+    """The reference's MinIndx/MaxIndx tie order in closed form (NaN-free images), derived
+    from cl-files/Statistics.cl:344-425 without simulating its tree: a halving tree in which
+    position p takes p + k only when strictly better settles two tied work-items at the level
+    of the lowest bit in which their lids differ, and the one with a 0 there wins; so among
+    the work-items holding the group's extreme the winner has the smallest bit-reversed lid
+    (lid = 16 ly + lx), and inside a work-item the smallest i (column 256 gxg + lx + 16 i).
+    Groups compare their extremes as float32 and the lowest group id wins ties
+    (StatisticsHelpers.cpp:105-145)."""
+    a = np.asarray(src)
+    H, W = a.shape
+    ngx = -(-W // 256)
+    ys, xs = np.mgrid[0:H, 0:W]
+    gid = (ys // 16) * ngx + xs // 256
+    lid = 16 * (ys % 16) + xs % 16
+    i = (xs % 256) // 16
+    val = a.astype(np.float64)                       # exact for every element type
+    best_group = {}
+    for g in np.unique(gid):
+        m = gid == g
+        ext = val[m].max() if want_max else val[m].min()
+        cand = m & (val == ext)
+        key = _bitrev8(lid[cand]) * 16 + i[cand]
+        k = np.argmin(key)
+        best_group[int(g)] = (np.float32(ext), int(xs[cand][k]), int(ys[cand][k]))
+    parts = [best_group[g] for g in sorted(best_group)]
+    fl = np.array([p[0] for p in parts], np.float32)
+    ext = fl.max() if want_max else fl.min()
+    g = int(np.flatnonzero(fl == ext)[0])
+    return float(parts[g][0]), parts[g][1], parts[g][2]
+
+
+def test_min_max_indx_reference_tie_order_known_answers():
+    """Hand-derived from the reference's kernel: pixel (16, 0) is work-item 0's i = 1 and
+    (1, 0) is work-item 1's i = 0; the tree's last step keeps position 0 -> (16, 0).  (0, 0)
+    starts at position 0 and wins every tie.  (5, 1) is lid 21 and (6, 0) lid 6: 21 = 10101b
+    and 6 = 00110b first differ in bit 0, where 6 has the 0 -> (6, 0).  Different groups: the
+    lower group id wins, (300, 0) (group 1) over (0, 16) (group ngx = 2)."""
+    def case(shape, pts, want_max=True, dtype=np.uint8):
+        img = np.zeros(shape, dtype) if want_max else np.full(shape, 9, dtype)
+        for x, y in pts:
+            img[y, x] = 9 if want_max else 0
+        return orc.min_max_indx(img, want_max)[1:]
+    assert case((4, 40), [(1, 0), (16, 0)]) == (16, 0)
+    assert case((4, 40), [(1, 0), (16, 0)], want_max=False) == (16, 0)
+    assert case((20, 300), [(0, 0), (17, 3), (299, 19)]) == (0, 0)
+    assert case((4, 40), [(5, 1), (6, 0)]) == (6, 0)
+    assert case((32, 512), [(0, 16), (300, 0)]) == (300, 0)
+    # no ties: the unique extreme
     rng = np.random.default_rng(12)
     src = rng.random((33, 47)).astype(np.float32)
     v, x, y = orc.min_max_indx(src, True)
     k = np.argmax(src)
     assert (y, x) == np.unravel_index(k, src.shape) and v == src.flat[k]
-    v, x, y = orc.min_max_indx(src, False)
-    k = np.argmin(src)
-    assert (y, x) == np.unravel_index(k, src.shape) and v == src.flat[k]
-    tie = np.zeros((4, 5), np.uint8)
-    tie[1, 3] = tie[2, 0] = 9
-    assert orc.min_max_indx(tie, True) == (9.0, 3, 1)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int8, np.uint16, np.int16, np.uint32,
+                                   np.int32, np.float32, np.float64])
+def test_min_max_indx_matches_closed_form(dtype):
+    """Tie-heavy images (few distinct values) of ragged sizes: the oracle's tree simulation
+    against the closed form above.  Wide 32-bit and F64 values also exercise the float
+    partials: values that differ below float precision tie between groups."""
+    rng = np.random.default_rng(40 + np.dtype(dtype).itemsize)
+    for shape in [(1, 1), (3, 17), (16, 256), (17, 257), (45, 530), (70, 300)]:
+        if dtype in (np.uint32, np.int32):
+            a = (2 ** 30 + rng.integers(0, 4, shape) * 16).astype(dtype)   # ties after rounding
+        elif dtype == np.float64:
+            a = 1.0 + rng.integers(0, 3, shape) * 2.0 ** -40
+        elif dtype == np.float32:
+            a = rng.integers(0, 3, shape).astype(np.float32) * np.float32(0.25)
+        else:
+            info = np.iinfo(dtype)
+            a = rng.integers(max(info.min, -2), min(info.max, 3), shape).astype(dtype)
+        a = a.astype(dtype)
+        for want_max in (True, False):
+            assert orc.min_max_indx(a, want_max) == _min_max_indx_closed_form(a, want_max), \
+                (shape, want_max)
+
+
+def test_min_max_indx_nan_follows_the_comparisons():
+    """A NaN is never taken; a NaN held by work-item 0's first pixel is never replaced, so the
+    result is NaN at (0, 0)."""
+    a = np.ones((20, 40), np.float32)
+    a[3, 7] = 5.0
+    a[1, 1] = np.nan
+    assert orc.min_max_indx(a, True) == (5.0, 7, 3)
+    a[0, 0] = np.nan
+    v, x, y = orc.min_max_indx(a, True)
+    assert np.isnan(v) and (x, y) == (0, 0)
 
 
 def test_sum_ref_exact_where_float_partials_are_exact():
