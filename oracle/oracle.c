@@ -6,7 +6,8 @@
  *
  * Contents (each function cites the reference code it restates):
  *   orc_prox_ref      reference-order proximity in REAL precision
- *                     (cl-files/ImageProximity.cl:44-466, cl-files/Buffers.h:58-84)
+ *                     (cl-files/ImageProximity.cl:44-466, cl-files/Buffers.h:58-84);
+ *                     orc_prox_ref_rows: the same on a list of output rows
  *   orc_prox_f64      same sums accumulated in FP64 (ground truth for float sources)
  *   orc_prox_i64      exact int64 sums for integer sources
  *   orc_integral_ref  reference summation tree of the 5-launch integral scan
@@ -177,6 +178,35 @@ int orc_prox_ref(int op, int type, int nch,
       #pragma omp parallel for schedule(dynamic, 1)
       for (int yy = y0; yy < y1; ++yy) {
          int y0_ = yy, y1_ = yy + 1;
+         { int y0 = y0_, y1 = y1_; PROX_BODY(float, fmaf, sqrtf, elem_f) }
+      }
+   }
+   return 0;
+}
+
+/* orc_prox_ref on the listed output rows only (the other rows of dst are left as they are):
+ * spread samples of large maps in one parallel call. */
+int orc_prox_ref_rows(int op, int type, int nch,
+                      const void *src, int W, int H, int sstep,
+                      const void *tpl, int tw, int th, int tstep,
+                      void *dst, int dstep, int contract, const int *rows, int nrows,
+                      int nthreads)
+{
+   if (op < 0 || op > 4 || type < 0 || type > T_F64 || nch < 1 || nch > 4) return -1;
+   for (int k = 0; k < nrows; ++k)
+      if (rows[k] < 0 || rows[k] >= H) return -1;
+   const int left = -(tw / 2), top = -(th / 2);
+   set_threads(nthreads);
+   if (type == T_F64) {
+      #pragma omp parallel for schedule(dynamic, 1)
+      for (int k = 0; k < nrows; ++k) {
+         int y0_ = rows[k], y1_ = rows[k] + 1;
+         { int y0 = y0_, y1 = y1_; PROX_BODY(double, fma, sqrt, elem_dd) }
+      }
+   } else {
+      #pragma omp parallel for schedule(dynamic, 1)
+      for (int k = 0; k < nrows; ++k) {
+         int y0_ = rows[k], y1_ = rows[k] + 1;
          { int y0 = y0_, y1 = y1_; PROX_BODY(float, fmaf, sqrtf, elem_f) }
       }
    }

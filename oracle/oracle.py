@@ -16,6 +16,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_BASE_PATH = os.path.join(_HERE, "_build", "libbaseline.so")
 
 # SImage::EDataType (reference include/SImage.h:36-46)
 U8, S8, U16, S16, U32, S32, F32, F64 = range(8)
@@ -43,6 +44,8 @@ def lib():
         vp, i32 = ctypes.c_void_p, ctypes.c_int
         L.orc_prox_ref.argtypes = [i32, i32, i32, vp, i32, i32, i32, vp, i32, i32, i32,
                                    vp, i32, i32, i32, i32, i32]
+        L.orc_prox_ref_rows.argtypes = [i32, i32, i32, vp, i32, i32, i32, vp, i32, i32, i32,
+                                        vp, i32, i32, vp, i32, i32]
         L.orc_prox_f64.argtypes = [i32, i32, i32, vp, i32, i32, i32, vp, i32, i32, i32, vp, i32]
         L.orc_prox_i64.argtypes = [i32, i32, i32, vp, i32, i32, i32, vp, i32, i32, i32, vp, i32]
         L.orc_integral_ref.argtypes = [i32, i32, vp, i32, i32, i32, i32, vp, i32, i32]
@@ -51,11 +54,45 @@ def lib():
         L.orc_min_max_indx.argtypes = [i32, i32, vp, i32, i32, i32,
                                        ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
-        for f in ("orc_prox_ref", "orc_prox_f64", "orc_prox_i64", "orc_integral_ref",
+        for f in ("orc_prox_ref", "orc_prox_ref_rows", "orc_prox_f64", "orc_prox_i64", "orc_integral_ref",
                   "orc_integral_exact", "orc_area_sum", "orc_min_max_indx"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
+
+
+_base = None
+
+
+def baseline_lib():
+    """The CPU baseline (oracle/baseline.c): timing only, never the checker."""
+    global _base
+    if _base is None:
+        if not os.path.exists(_BASE_PATH):
+            build()
+        L = ctypes.CDLL(_BASE_PATH)
+        vp, i32 = ctypes.c_void_p, ctypes.c_int
+        L.base_cc_norm.argtypes = [vp, i32, i32, i32, vp, i32, i32, i32, vp, i32, i32, i32, i32]
+        L.base_cc_norm.restype = i32
+        L.base_isa.argtypes = []
+        L.base_isa.restype = ctypes.c_char_p
+        _base = L
+    return _base
+
+
+def baseline_cc_norm(src, tpl, rows=None, nthreads=0, out=None):
+    """CrossCorr_Norm of a 1-channel F32 image by the CPU baseline (rows=(y0, y1) restricts
+    the output rows computed).  Returns (map, instruction set used)."""
+    sp, W, H, C, ss, t = _desc(src)
+    tp, tw, th, tc, ts, tt = _desc(tpl)
+    assert C == 1 and tc == 1 and t == F32 and tt == F32
+    if out is None:
+        out = np.zeros((H, W), np.float32)
+    y0, y1 = (0, H) if rows is None else rows
+    r = baseline_lib().base_cc_norm(sp, W, H, ss, tp, tw, th, ts, out.ctypes.data,
+                                    out.strides[0], y0, y1, nthreads)
+    assert r == 0
+    return out, baseline_lib().base_isa().decode()
 
 
 def _desc(a):
@@ -88,6 +125,21 @@ def prox_ref(op, src, tpl, contract=True, rows=None, nthreads=0):
                            out.strides[0], 1 if contract else 0, y0, y1, nthreads)
     assert r == 0
     return out
+
+
+def prox_ref_rows(op, src, tpl, rows, contract=True, nthreads=0):
+    """prox_ref on the listed output rows only: returns an array of shape (len(rows), W[, C])."""
+    sp, W, H, C, ss, t = _desc(src)
+    tp, tw, th, tc, ts, tt = _desc(tpl)
+    assert tc == C and tt == t
+    real = np.float64 if t == F64 else np.float32
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    out = np.zeros((H, W, C) if src.ndim == 3 else (H, W), dtype=real)
+    r = lib().orc_prox_ref_rows(op, t, C, sp, W, H, ss, tp, tw, th, ts, out.ctypes.data,
+                                out.strides[0], 1 if contract else 0, rows.ctypes.data,
+                                len(rows), nthreads)
+    assert r == 0
+    return out[rows]
 
 
 def prox_f64(op, src, tpl, nthreads=0):

@@ -7,7 +7,9 @@ here, in the test harness only.  Checks:
   (torch conv2d on the CPU), for several template shapes including even sizes;
 * the 1024x1024 / 31x31 workload: against FP64 central finite differences (h = 1e-3) of an
   FP64 forward (scipy FFT correlation) at 64 seeded coordinates (32 source pixels, 32
-  template taps), with L = sum(w1 * NCC) + sum(w2 * SSD), w ~ N(0, 1).
+  template taps), with L = sum(w1 * NCC) + sum(w2 * SSD), w ~ N(0, 1); and all 16 images'
+  full gradients against an FP64 reference gradient (torch float64 autograd through an FFT
+  forward).
 """
 import numpy as np
 import pytest
@@ -195,3 +197,65 @@ def test_diff_1024_finite_differences(ocip):
         fd = (local(h) - local(-h)) / (2 * h)
         got = dS_all[k][i, j]
         assert abs(got - fd) <= 1e-3 * abs(fd) + 2e-3, (i, j, got, fd)
+
+
+def ref_grad_fft(s_np, t_np, w1, w2):
+    """FP64 reference gradient of L = sum(w1 * NCC) + sum(w2 * SSD): torch float64 autograd
+    (CPU) through a forward whose correlations are taken with FFTs (zero padding, template
+    origin at (-tw/2, -th/2)); independent of the library's gradient formulas."""
+    s = torch.tensor(s_np.astype(np.float64), requires_grad=True)
+    t = torch.tensor(t_np.astype(np.float64), requires_grad=True)
+    th, tw = t.shape
+    H, W = s.shape
+    cy, cx = th // 2, tw // 2
+    P = torch.nn.functional.pad(s, (cx, tw - 1 - cx, cy, th - 1 - cy))
+    Hp, Wp = P.shape
+
+    def corr(A, B):
+        # circular cross-correlation; rows < H and columns < W never wrap
+        FA = torch.fft.rfft2(A)
+        FB = torch.fft.rfft2(B, s=(Hp, Wp))
+        return torch.fft.irfft2(FA * FB.conj(), s=(Hp, Wp))[:H, :W]
+
+    cc = corr(P, t)
+    ss = corr(P * P, torch.ones_like(t))
+    tt = (t * t).sum()
+    ncc = cc / torch.sqrt(ss * tt)
+    ssd = ss - 2 * cc + tt
+    ((torch.from_numpy(w1.astype(np.float64)) * ncc).sum() +
+     (torch.from_numpy(w2.astype(np.float64)) * ssd).sum()).backward()
+    return s.grad.numpy(), t.grad.numpy()
+
+
+def test_diff_1024_all_images_against_fp64_reference(ocip):
+    """All 16 images of the 1024x1024 / 31x31 workload: every full source-gradient map, and
+    the template gradient summed over the batch, within 2e-4 of the largest component of the
+    FP64 reference gradient."""
+    from openclipp_amd import workloads
+    srcs, tpl, w_ncc, w_ssd = workloads.diff_1024x16()
+    T = ocip.Image(tpl)
+    dS = ocip.Image(shape=srcs[0].shape, dtype=np.float32)
+    dT = ocip.Image(shape=tpl.shape, dtype=np.float32)
+    dT_got = np.zeros(tpl.shape, np.float64)
+    dT_want = np.zeros(tpl.shape, np.float64)
+    worst_s = 0.0
+    for k in range(len(srcs)):
+        say("diff_1024 fp64 reference, image", k)
+        S, G1, G2 = ocip.Image(srcs[k]), ocip.Image(w_ncc[k]), ocip.Image(w_ssd[k])
+        ocip.check(ocip.lib.ocipCrossCorr_NormBackward(S.handle, T.handle, G1.handle, dS.handle,
+                                                       dT.handle))
+        ds, dt = dS.read().astype(np.float64), dT.read().astype(np.float64)
+        ocip.check(ocip.lib.ocipSqrDistanceBackward(S.handle, T.handle, G2.handle, dS.handle,
+                                                    dT.handle))
+        ds += dS.read()
+        dT_got += dt + dT.read()
+        want_s, want_t = ref_grad_fft(srcs[k], tpl, w_ncc[k], w_ssd[k])
+        dT_want += want_t
+        err = np.max(np.abs(ds - want_s)) / np.abs(want_s).max()
+        worst_s = max(worst_s, float(err))
+        assert err <= 2e-4, (k, err)
+        for i in (S, G1, G2):
+            i.release()
+    err_t = np.max(np.abs(dT_got - dT_want)) / np.abs(dT_want).max()
+    say("diff_1024 relative error: dS", worst_s, "dT", float(err_t))
+    assert err_t <= 2e-4

@@ -9,8 +9,9 @@
 * The batch is recorded into a HIP graph on its first call; later calls replay it.  Replays
   are checked one by one, with host work, new template contents sent between replays, and
   synchronisations in between.
-* The 2048x2048 / 256-template workload: sampled pixels against FP64 and the crop location
-  of every template is the arg-max of its NCC map.
+* The 2048x2048 / 256-template workload, CrossCorr_Norm and SqrDistance_Norm: sampled pixels
+  against FP64 and the crop location of every 8th template is the arg-max (arg-min) of its
+  map.
 """
 import ctypes
 
@@ -145,11 +146,13 @@ def test_batch_argument_errors(ocip):
     assert L.ocipSqrDistance_NormBatch(S8.handle, ocip.handles([T8]), ocip.handles([D]), 1, 0) == ocip.CL_INVALID_VALUE
 
 
-def test_batch_2048_256_templates(ocip):
+def _batch_2048_check(got, tpls, offs, ssd):
+    """Every 8th template: the crop is the arg-max (NCC) / arg-min (SSD_Norm) of its map, and
+    63 random pixels plus the crop's are within tolerance of the FP64 value: 1e-3 for NCC,
+    2e-3 for SSD_Norm = (SS - 2 CC + TT) / sqrt(SS TT), whose only inexact term is twice the
+    NCC's.  Returns the largest error seen."""
     from openclipp_amd import workloads
-    src, tpls, offs = workloads.batch_2048()
-    say("batch_2048", len(tpls))
-    got, _ = run_batch(ocip, ocip.lib.ocipCrossCorr_NormBatch, src, tpls, "bf16")
+    src = workloads.batch_2048()[0]
     rng = np.random.default_rng(0)
     P = np.pad(src.astype(np.float64), 96)
     worst = 0.0
@@ -157,13 +160,34 @@ def test_batch_2048_256_templates(ocip):
         t, (ox, oy), g = tpls[k], offs[k], got[k]
         th, tw = t.shape
         y, x = oy + th // 2, ox + tw // 2
-        assert np.unravel_index(np.nanargmax(g), g.shape) == (y, x), k
+        at = np.nanargmin(g) if ssd else np.nanargmax(g)
+        assert np.unravel_index(at, g.shape) == (y, x), k
         ys = np.concatenate([rng.integers(0, 2048, 63), [y]])
         xs = np.concatenate([rng.integers(0, 2048, 63), [x]])
         t64 = t.astype(np.float64)
         win = np.stack([P[yy + 96 - th // 2: yy + 96 - th // 2 + th, xx + 96 - tw // 2: xx + 96 - tw // 2 + tw]
                         for xx, yy in zip(xs, ys)])
-        want = (win * t64).sum(axis=(1, 2)) / np.sqrt((win * win).sum(axis=(1, 2)) * (t64 * t64).sum())
+        den = np.sqrt((win * win).sum(axis=(1, 2)) * (t64 * t64).sum())
+        if ssd:
+            want = ((win - t64) ** 2).sum(axis=(1, 2)) / den
+        else:
+            want = (win * t64).sum(axis=(1, 2)) / den
         worst = max(worst, float(np.max(np.abs(g[ys, xs] - want))))
-    say("batch_2048 max abs NCC error", worst)
-    assert worst <= 1e-3
+    assert worst <= (2e-3 if ssd else 1e-3)
+    return worst
+
+
+def test_batch_2048_256_templates(ocip):
+    from openclipp_amd import workloads
+    src, tpls, offs = workloads.batch_2048()
+    say("batch_2048", len(tpls))
+    got, _ = run_batch(ocip, ocip.lib.ocipCrossCorr_NormBatch, src, tpls, "bf16")
+    say("batch_2048 max abs NCC error", _batch_2048_check(got, tpls, offs, ssd=False))
+
+
+def test_batch_2048_256_templates_ssd_norm(ocip):
+    from openclipp_amd import workloads
+    src, tpls, offs = workloads.batch_2048()
+    say("batch_2048 SqrDistance_Norm", len(tpls))
+    got, _ = run_batch(ocip, ocip.lib.ocipSqrDistance_NormBatch, src, tpls, "bf16")
+    say("batch_2048 max abs SSD_Norm error", _batch_2048_check(got, tpls, offs, ssd=True))

@@ -278,6 +278,24 @@ def test_integer_4096_bands(ocip):
             assert got[y, x] == np.float32((noise * noise).sum())
 
 
+def test_integer_4096_float_maps_reference_order(ocip):
+    """The U8 workload's float maps (CrossCorr, CrossCorr_Norm, SqrDistance_Norm) against the
+    reference's own float-order sums (REAL = float, ImageProximity.cl:290-412) on 256 bands of
+    2 rows spread over the image: within 1e-5 relative.  The library's sums are exact, so the
+    difference is the reference's float rounding (at most 7e-6 on these rows)."""
+    from openclipp_amd import workloads
+    src, tpl, _ = workloads.integer_4096()
+    y0s = np.linspace(0, 4094, 256).astype(int)
+    rows = np.unique(np.concatenate([y0s, y0s + 1]))
+    for op in ["CrossCorr", "CrossCorr_Norm", "SqrDistance_Norm"]:
+        say("integer_4096 reference order", op)
+        got = run_op(ocip, op, src, tpl)[rows].astype(np.float64)
+        want = orc.prox_ref_rows(OP_CODE[op], src, tpl, rows).astype(np.float64)
+        rel = np.abs(got - want) / np.abs(want)
+        say("  max relative difference", float(rel.max()))
+        assert rel.max() <= 1e-5, op
+
+
 def test_precision_4096_plain_bands(ocip):
     from openclipp_amd import workloads
     src, tpl, (ox, oy) = workloads.precision_4096()
@@ -289,6 +307,12 @@ def test_precision_4096_plain_bands(ocip):
     for y0, y1 in [(0, 2), (2047, 2049), (4094, 4096)]:
         want = band_oracle(lambda c: orc.prox_ref(orc.CC, c, tpl, nthreads=8), src, tpl, y0, y1)
         np.testing.assert_array_equal(cc[y0:y1], want)
+    # CrossCorr_Norm on 64 rows spread over the image (the first and last two included)
+    # against the reference-order map: within 1e-5 relative (1e-6 absolute floor)
+    rows = np.unique(np.concatenate([[0, 1, 4094, 4095], np.linspace(2, 4093, 60).astype(int)]))
+    want = orc.prox_ref_rows(orc.CC_NORM, src, tpl, rows).astype(np.float64)
+    got = ncc[rows].astype(np.float64)
+    assert np.all(np.abs(got - want) <= 1e-5 * np.abs(want) + 1e-6)
 
 
 def test_golden_fixtures_through_the_abi(ocip):

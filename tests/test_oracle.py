@@ -352,3 +352,18 @@ def test_sum_exact_against_fsum():
     assert orc.sum_exact(a.astype(np.float32), True)[0] == math.fsum(
         float(t) ** 2 for t in a.astype(np.float32).ravel())
     assert orc.sum_exact(np.array([[3, -4]], np.int16), True)[0] == 25.0
+
+
+def test_cpu_baseline_is_the_reference_order():
+    """The benchmark's CPU baseline (oracle/baseline.c, vectorised across adjacent outputs)
+    computes the reference's CrossCorr_Norm bit for bit: the same map as the reference-order
+    restatement, on ragged sizes, a 1x1 template and windows over every border."""
+    rng = np.random.default_rng(71)
+    for (H, W), (th, tw) in [((50, 70), (7, 9)), ((130, 200), (16, 16)), ((97, 301), (64, 64)),
+                             ((33, 65), (1, 1)), ((20, 129), (20, 12))]:
+        src = rng.random((H, W)).astype(np.float32)
+        tpl = rng.random((th, tw)).astype(np.float32)
+        got, _ = orc.baseline_cc_norm(src, tpl)
+        np.testing.assert_array_equal(got, orc.prox_ref(orc.CC_NORM, src, tpl, contract=True))
+        band, _ = orc.baseline_cc_norm(src, tpl, rows=(3, H - 2))
+        np.testing.assert_array_equal(band[3:H - 2], got[3:H - 2])
