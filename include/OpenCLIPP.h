@@ -47,6 +47,7 @@ typedef cl_bitfield cl_mem_flags;
 #  define CL_INVALID_IMAGE_SIZE            -40
 #  define CL_INVALID_PROGRAM               -44
 #  define CL_INVALID_OPERATION             -59
+#  define CL_INVALID_BUFFER_SIZE           -61
 #  define CL_DEVICE_TYPE_DEFAULT      (1 << 0)
 #  define CL_DEVICE_TYPE_CPU          (1 << 1)
 #  define CL_DEVICE_TYPE_GPU          (1 << 2)
@@ -184,6 +185,18 @@ ocipError ocip_API ocipCrossCorrEx(ocipImage Source, ocipImage Template, ocipIma
                                    int Precision);
 ocipError ocip_API ocipCrossCorr_NormEx(ocipImage Source, ocipImage Template, ocipImage Dest,
                                         int Precision);
+
+/* ocipCrossCorr_NormEx that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (serving loops, code under HIP graph capture,
+ * harnesses that forbid hipMalloc).  MFMA precisions only (ocipPrecisionF32 gives
+ * CL_INVALID_VALUE), templates up to 513 columns (wider ones: CL_INVALID_VALUE) whose
+ * fragments fit an 8 MiB per-GPU scratch array held until the process ends (bigger ones:
+ * CL_INVALID_BUFFER_SIZE; a 64x64 BF16 template needs about 0.6 MiB).  No autotuning: the
+ * kernel variant tuned for the same shape by an earlier ocipCrossCorr_NormEx call, else a
+ * fixed one, so its map equals ocipCrossCorr_NormEx's once that call has run.  Calls that
+ * overlap on one GPU share the array and race. */
+ocipError ocip_API ocipCrossCorr_NormExNoAlloc(ocipImage Source, ocipImage Template,
+                                               ocipImage Dest, int Precision);
 
 /* Many templates of any sizes against one F32 1-channel source: Dests[i] receives the map of
  * Templates[i].  One persistent launch (replayed from a HIP graph) covers the whole batch.

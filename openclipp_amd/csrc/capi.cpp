@@ -328,6 +328,7 @@ ocipError ocip_API ocipInitialize(ocipContext *ContextPtr, const char *Preferred
          if (p.gcnArchName[0]) c->name += std::string(" (") + p.gcnArchName + ")";
          c->num_cus = p.multiProcessorCount;
          hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream");
+         noalloc_scratch_init(c->device);
       } catch (...) {
          delete c;
          throw;
@@ -587,6 +588,40 @@ ocipError ocip_API ocipCrossCorr_NormEx(ocipImage Source, ocipImage Template, oc
                                         int Precision)
 {
    return guarded([&] { run_proximity(kCCNorm, Source, Template, Dest, Precision); });
+}
+
+// ocipCrossCorr_NormEx without any allocation: MFMA precisions, templates whose fragments fit
+// the per-GPU scratch array (noalloc.hip), no autotuning (the variant tuned earlier for the
+// shape, else the row-pair kernel).  Overlapping calls on one GPU race on that array.
+ocipError ocip_API ocipCrossCorr_NormExNoAlloc(ocipImage Source, ocipImage Template,
+                                               ocipImage Dest, int Precision)
+{
+   return guarded([&] {
+      Context &c = current();
+      Image &S = img(Source), &T = img(Template), &D = img(Dest);
+      throw_if(check::proximity(S.desc, T.desc, D.desc), "proximity arguments");
+      throw_if(check::precision(S.desc, Precision), "precision");
+      if (Precision == ocipPrecisionF32)
+         fail(CL_INVALID_VALUE, "no-allocation entry point: MFMA precisions only");
+      const int tw = T.desc.Width, th = T.desc.Height;
+      if (tw == 0 || th == 0) fail(CL_INVALID_VALUE, "empty template");
+      if (crosscorr_mfma_needs_integral(tw))
+         fail(CL_INVALID_VALUE, "no-allocation entry point: template wider than 513 columns");
+      const size_t need = kNoAllocSumsBytes + crosscorr_mfma_work_bytes(
+                              Precision, S.desc.Width, S.desc.Height, tw, th);
+      if (need > kNoAllocScratchBytes)
+         fail(CL_INVALID_BUFFER_SIZE, "no-allocation entry point: template needs more than "
+                                      "the 8 MiB scratch array");
+      same_context(c, S); same_context(c, T); same_context(c, D);
+      set_device(c);
+      send_if_needed(c, S);
+      send_if_needed(c, T);
+      char *base = static_cast<char *>(noalloc_scratch(c.device));
+      launch_crosscorr_mfma(c.stream, Precision, true, view(S), view(T),
+                            reinterpret_cast<double *>(base), nullptr, static_cast<float *>(D.dev),
+                            D.desc.Step, base + kNoAllocSumsBytes, false);
+      D.in_device = true;
+   });
 }
 
 ocipError ocip_API ocipCrossCorr_NormBatch(ocipImage Source, const ocipImage *Templates,

@@ -670,7 +670,8 @@ static void launch_ncc_scale(hipStream_t s, const View &S, int tw, int th, const
 }
 
 void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &S, const View &T,
-                           double *tsums, const double *integ, float *dst, int dstep, void *work)
+                           double *tsums, const double *integ, float *dst, int dstep, void *work,
+                           bool may_tune)
 {
    const int tw = T.width, th = T.height;
    if (tw <= 0 || th <= 0) fail(CL_INVALID_VALUE, "empty template");
@@ -707,6 +708,9 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
          for (int c : cands)
             if (c == known->second) chosen = c;
    }
+   // without tuning: the row-pair kernel with 96-column tiles (within noise of the fastest
+   // variant from 32x32 to 64x64 templates), or crosscorr_mfma for split BF16
+   if (!chosen && !may_tune) chosen = pair_ok ? 16 : 4;
 
    char *wk = static_cast<char *>(work);
    const size_t nfrag = (size_t)nub * th * KC;

@@ -188,13 +188,23 @@ void launch_crosscorr_f32_anchored(hipStream_t s, const View &M, const float *ts
 
 // MFMA cross-correlation (F32 1-channel source and template), precision 1..3.
 // The template's fragments and sum of squares (tsums[0]) are built from T in the call.
+// may_tune = false: no autotuning (no timing events); the variant tuned earlier for this
+// shape, else a fixed default.
 void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &src,
                            const View &T, double *tsums, const double *integ, float *dst,
-                           int dst_step, void *work);
+                           int dst_step, void *work, bool may_tune = true);
 // The single-template MFMA Norm path forms its denominators from the source image; only
 // templates wider than 513 columns read the FP64 sum-of-squares integral instead.
 bool crosscorr_mfma_needs_integral(int tw);
 size_t crosscorr_mfma_work_bytes(int precision, int width, int height, int tw, int th);
+
+// Scratch of the entry point that allocates nothing (noalloc.hip): a device array of
+// kNoAllocScratchBytes per GPU.  noalloc_scratch_init takes its address on `device` (which
+// must be current) once; noalloc_scratch returns that address.
+constexpr size_t kNoAllocScratchBytes = (size_t)8 << 20;
+constexpr size_t kNoAllocSumsBytes = 256;   // template sums ahead of the fragments
+void noalloc_scratch_init(int device);
+void *noalloc_scratch(int device);
 
 // Batched MFMA CrossCorr_Norm (ssd = false) or SqrDistance_Norm (ssd = true) of F32 1-channel
 // templates against one F32 1-channel source, recorded into a HIP graph on first use and

@@ -94,6 +94,7 @@ SIGNATURES = {
     "ocipCrossCorrFFT_Norm": [_vp, _vp, _vp, _vp],
     "ocipCrossCorrEx": [_vp, _vp, _vp, ctypes.c_int],
     "ocipCrossCorr_NormEx": [_vp, _vp, _vp, ctypes.c_int],
+    "ocipCrossCorr_NormExNoAlloc": [_vp, _vp, _vp, ctypes.c_int],
     "ocipCrossCorr_NormBatch": [_vp, _ptr_vp, _ptr_vp, ctypes.c_uint, ctypes.c_int],
     "ocipSqrDistance_NormBatch": [_vp, _ptr_vp, _ptr_vp, ctypes.c_uint, ctypes.c_int],
     "ocipCrossCorr_NormBackward": [_vp, _vp, _vp, _vp, _vp],
