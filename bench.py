@@ -95,7 +95,15 @@ def batch_bench(ocip, steps):
         fn = lambda: ocip.lib.ocipCrossCorr_NormBatch(S.handle, th, dh, len(Ts), prec)
         ms, kms = time_steps(ocip, fn, n, 1, "crosscorr_mfma_batch" if prec else "prox_tile_f32")
         out[name] = {"value": round(tpix / (ms * 1e-3) / 1e9, 3), "ms_per_step": round(ms, 3)}
+        if prec:
+            ach = 2.0 * tpix / (ms * 1e-3) / 1e12
+            out[name]["achieved_tflops"] = round(ach, 1)
+            out[name]["frac_of_bf16_peak"] = round(ach / PEAK_TFLOPS["bf16"], 4)
     out["speedup_bf16_vs_f32"] = round(out["bf16"]["value"] / out["f32"]["value"], 3)
+    # the same batch, SqrDistance_Norm (BF16 MFMA, SS - 2 CC + TT)
+    fn = lambda: ocip.lib.ocipSqrDistance_NormBatch(S.handle, th, dh, len(Ts), 1)
+    ms, _ = time_steps(ocip, fn, steps, 1, "crosscorr_mfma_batch")
+    out["ssd_norm_bf16"] = {"value": round(tpix / (ms * 1e-3) / 1e9, 3), "ms_per_step": round(ms, 3)}
     for i in [S] + Ts + Ds:
         i.release()
     return out
@@ -197,11 +205,18 @@ def configs_bench(ocip, steps):
         return ocip.CL_SUCCESS
 
     ms, kms = time_steps(ocip, grads, max(2, steps // 8), 1, "backward")
+    # algorithmic work: 2 FLOP per tpix per gradient map, two maps (source and template) per
+    # op, two ops per image
+    gflop = 2.0 * 2 * 2 * n * W * H * tw * th / 1e9
     out["diff_1024x16"] = {
         "workload": f"{n} x {W}x{H} F32 U[0,1), {tw}x{th} N(0,1) template: CrossCorr_Norm and "
                     f"SqrDistance gradients wrt images and template, {n} images per step",
         "ms_per_step": round(ms, 4),
-        "gradient_gpix_tpix_per_s": round(2 * 2 * n * W * H * tw * th / (ms * 1e-3) / 1e9, 1)}
+        "gradient_gpix_tpix_per_s": round(2 * 2 * n * W * H * tw * th / (ms * 1e-3) / 1e9, 1),
+        "roofline": {"bound": "valu", "achieved": round(gflop / ms, 2),
+                     "peak": PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
+                     "frac": round(gflop / ms / PEAK_TFLOPS["f32"], 4),
+                     "note": "whole step (every kernel of the 32 backward calls)"}}
     for i in Ss + G1 + G2 + [T, dS, dT]:
         i.release()
     return out
@@ -319,6 +334,21 @@ def main():
     err = np.abs(got - want) if got is not None else np.abs(got32 - want)
     err32 = np.abs(got32 - want)
 
+    # the other MFMA precisions on the same workload: FP16 and split BF16 (hi + lo, three
+    # products: the F32-accuracy figure)
+    others = {}
+    for p in ("f16", "bf16x3"):
+        if p == prec:
+            continue
+        ms_p, kms_p = time_steps(ocip, step(PREC_CODE[p]), max(3, args.steps // 2), 1, DOMINANT[p])
+        e = np.abs(D.read()[ys, xs].astype(np.float64) - want)
+        ach_p = 2.0 * tpix / (kms_p * 1e-3) / 1e12 if kms_p else None
+        others[p] = {"value": round(tpix / (ms_p * 1e-3) / 1e9, 3), "ms_per_step": round(ms_p, 4),
+                     "kernel_ms": round(kms_p, 4) if kms_p else None,
+                     "achieved_tflops": round(ach_p, 1) if ach_p else None,
+                     "frac": round(ach_p / PEAK_TFLOPS[p], 4) if ach_p else None,
+                     "max_abs_err": float(e.max()), "mean_abs_err": float(e.mean())}
+
     flops = 2.0 * tpix
     achieved = flops / (kms * 1e-3) / 1e12 if kms else None
     peak = PEAK_TFLOPS[prec]
@@ -344,6 +374,7 @@ def main():
                       "achieved_tflops": round(flops / (kms32 * 1e-3) / 1e12, 2) if kms32 else None,
                       "max_abs_err": float(err32.max()), "mean_abs_err": float(err32.mean())},
         "speedup_vs_plain_f32": round(value / value32, 3),
+        "other_precisions": others,
         "crosscorr_same_precision": {"value": round(tpix / (ms_cc * 1e-3) / 1e9, 3),
                                      "ms_per_step": round(ms_cc, 4),
                                      "kernel_ms": round(kms_cc, 4) if kms_cc else None},
