@@ -178,10 +178,29 @@ def configs_bench(ocip, steps):
     D = ocip.Image(shape=(H, W), dtype=np.float32)
     integer = {"workload": f"{W}x{H} U8 blurred field, {tw}x{th} crop + U{{-3..3}}, exact integer sums"}
     for op in ocip.PROXIMITY_OPS:
+        # the exact-integer MFMA kernel (BF16 products) for the cross-term ops, the VALU kernel
+        # for AbsDistance
+        kname = "prox_tile_int" if op == "AbsDistance" else "crosscorr_mfma_int"
         ms, kms = time_steps(ocip, lambda: ocip.proximity(op, S, T, D), max(3, steps // 2), 1,
-                             "prox_tile_int")
-        integer[op] = {"ms_per_step": round(ms, 4), "kernel_ms": round(kms, 4) if kms else None,
-                       "gpix_tpix_per_s": round(W * H * tw * th / (ms * 1e-3) / 1e9, 1)}
+                             kname)
+        row = {"ms_per_step": round(ms, 4), "kernel": kname,
+               "kernel_ms": round(kms, 4) if kms else None,
+               "gpix_tpix_per_s": round(W * H * tw * th / (ms * 1e-3) / 1e9, 1)}
+        if kms and op != "AbsDistance":
+            # the kernel's algorithmic work is the cross term, 2 ops per tpix, on BF16 MFMA
+            ach = 2.0 * W * H * tw * th / (kms * 1e-3) / 1e12
+            row["roofline"] = {"bound": "mfma", "achieved": round(ach, 1),
+                               "peak": PEAK_TFLOPS["bf16"], "unit": "TFLOP/s",
+                               "frac": round(ach / PEAK_TFLOPS["bf16"], 4)}
+        elif kms:
+            # v_sad_u8: 4 taps (3 ops each) per lane and instruction at the non-packed VALU
+            # instruction rate (half the 157.3 TFLOP/s packed-FMA figure)
+            ach = 3.0 * W * H * tw * th / (kms * 1e-3) / 1e12
+            peak_sad = 3 * 4 * PEAK_TFLOPS["f32"] / 2
+            row["roofline"] = {"bound": "valu", "achieved": round(ach, 1),
+                               "peak": round(peak_sad, 1), "unit": "TOP/s",
+                               "frac": round(ach / peak_sad, 4)}
+        integer[op] = row
     out["integer_4096"] = integer
     for i in (S, T, D):
         i.release()

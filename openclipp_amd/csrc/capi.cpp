@@ -167,6 +167,14 @@ void run_proximity(int op, ocipImage hs, ocipImage ht, ocipImage hd, int precisi
    // SqrDistance on 8-bit sources is formed from the cross term and the window sums of
    // squares, which come from the integral as for the Norm ops
    const bool ssd8 = op == kSSD && (type == SImage::U8 || type == SImage::S8);
+   // 8-bit integer images: exact sums on BF16 MFMA where the path applies
+   if (precision == ocipPrecisionF32 && int_mfma_ok(op, view(S), view(T))) {
+      c.stats.ensure(int_mfma_work_bytes(tw, th));
+      launch_int_mfma(c.stream, op, view(S), view(T), c.tmpl.sums.as<double>(),
+                      static_cast<float *>(D.dev), D.desc.Step, c.stats.ptr);
+      D.in_device = true;
+      return;
+   }
    for (int ch = 0; ch < nch; ++ch) {
       const bool mfma = precision != ocipPrecisionF32;
       const bool need_integ = norm ? !mfma || crosscorr_mfma_needs_integral(tw) : ssd8 && !mfma;

@@ -793,6 +793,179 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
    if (norm) launch_ncc_scale(s, S, tw, th, tsums, integ, dst, dstep);
 }
 
+// ======================================================================= exact integer sums
+// 8-bit integer images through the row-pair kernel (kModeInt): the exact cross term of
+// CrossCorr, CrossCorr_Norm, SqrDistance and SqrDistance_Norm on BF16 MFMA, and the window
+// sums of s^2 from ss_box8.  Limits: templates up to 97 x 97 (one template column block, and
+// the rows ss_box8 stages), one row's products below 2^24 (the flush period is at least one
+// step) and the whole sum below 2^31 (int32), the LDS tiles.  OCIP_INT_MFMA=0 turns the
+// path off (the VALU kernel prox_tile_int then computes the same maps).
+static int int_max_product(int type) { return type == SImage::U8 ? 255 * 255 : 128 * 128; }
+
+// Window sums of s^2 of an 8-bit image, exact in int32, into the map `dst` (as int32 bits):
+// SS(x, y) = sum over the template window of (x, y) (origin (-tw/2, -th/2), zero outside the
+// image).  One workgroup per 128 x 64 outputs: the squares of the source region (127 + th
+// rows by 63 + tw columns, th and tw <= 97) are staged in LDS as 16-bit integers
+// (255^2 < 2^16; wave = rows, lane = columns, batches of 24 loads); a vertical pass (one task per 64-column chunk and half of the output rows:
+// lane = column) forms the half's first column sum over th rows and slides it down (adding
+// the row entering, subtracting the row leaving: integer arithmetic, exact in any order) into
+// an int32 LDS array; a horizontal pass (thread = output row and 32 columns) slides along the
+// rows the same way and stores 16-byte groups where the map's rows allow.
+constexpr int kBoxRows = 128, kBoxCols = 64;
+
+__global__ __launch_bounds__(256) void ss_box8(const unsigned char *__restrict__ src, int W,
+                                               int H, int sstep, int tw, int th, int is_signed,
+                                               char *__restrict__ dst, int dstep, int tiles_x,
+                                               int v4)
+{
+   extern __shared__ int box_lds[];
+   const int ncols = kBoxCols - 1 + tw, nrows = kBoxRows - 1 + th;
+   unsigned short *sq = reinterpret_cast<unsigned short *>(box_lds);        // nrows x ncols
+   int *vs = box_lds + (nrows * ncols + 1) / 2;                              // 128 x ncols
+   const int bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
+   const int x0 = bx * kBoxCols, y0 = by * kBoxRows;
+   const int c0 = x0 - tw / 2, r0 = y0 - th / 2;
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   // eight rows by up to three 64-column chunks per wave and batch: 24 loads in flight
+   for (int rbase = wave; rbase < nrows; rbase += 32) {
+      int v[8][3];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+         const int r = rbase + 4 * k, sy = r0 + r;
+         const bool rowin = r < nrows && sy >= 0 && sy < H;
+         const unsigned char *srow = src + (size_t)(rowin ? sy : 0) * sstep;
+#pragma unroll
+         for (int j = 0; j < 3; ++j) {
+            const int c = lane + 64 * j, sx = c0 + c;
+            v[k][j] = 0;
+            if (rowin && c < ncols && sx >= 0 && sx < W)
+               v[k][j] = is_signed ? (int)(signed char)srow[sx] : (int)srow[sx];
+         }
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+         const int r = rbase + 4 * k;
+#pragma unroll
+         for (int j = 0; j < 3; ++j) {
+            const int c = lane + 64 * j;
+            if (r < nrows && c < ncols) sq[r * ncols + c] = (unsigned short)(v[k][j] * v[k][j]);
+         }
+      }
+   }
+   __syncthreads();
+   const int nchunks = (ncols + 63) / 64;
+   for (int task = wave; task < 2 * nchunks; task += 4) {
+      const int c = (task >> 1) * 64 + lane, rb = (task & 1) * (kBoxRows / 2);
+      if (c >= ncols) continue;
+      int v = 0;
+#pragma unroll 8
+      for (int i = 0; i < th; ++i) v += sq[(rb + i) * ncols + c];
+      vs[rb * ncols + c] = v;
+#pragma unroll 9
+      for (int r = rb + 1; r < rb + kBoxRows / 2; ++r) {
+         v += (int)sq[(r + th - 1) * ncols + c] - (int)sq[(r - 1) * ncols + c];
+         vs[r * ncols + c] = v;
+      }
+   }
+   __syncthreads();
+   const int r = threadIdx.x >> 1, q = (threadIdx.x & 1) * 32;
+   const int y = y0 + r;
+   if (y >= H) return;
+   const int *vrow = vs + r * ncols;
+   int h = 0;
+#pragma unroll 8
+   for (int k = 0; k < tw; ++k) h += vrow[q + k];
+   int out[32];
+   out[0] = h;
+#pragma unroll
+   for (int j = 1; j < 32; ++j) {
+      h += vrow[q + j + tw - 1] - vrow[q + j - 1];
+      out[j] = h;
+   }
+   int *orow = reinterpret_cast<int *>(dst + (size_t)y * dstep) + x0 + q;
+   if (v4 && x0 + q + 32 <= W) {
+#pragma unroll
+      for (int j = 0; j < 32; j += 4)
+         *reinterpret_cast<int4 *>(orow + j) = make_int4(out[j], out[j + 1], out[j + 2], out[j + 3]);
+   } else {
+#pragma unroll
+      for (int j = 0; j < 32; ++j)
+         if (x0 + q + j < W) orow[j] = out[j];
+   }
+}
+
+static size_t ss_box8_lds(int tw, int th)
+{
+   const size_t ncols = kBoxCols - 1 + tw, nrows = kBoxRows - 1 + th;
+   return (nrows * ncols + 1) / 2 * 4 + (size_t)kBoxRows * ncols * 4;
+}
+
+bool int_mfma_ok(int op, const View &S, const View &T)
+{
+   const char *e = getenv("OCIP_INT_MFMA");
+   if ((e && e[0] == '0') || op == kSAD) return false;
+   if ((S.type != SImage::U8 && S.type != SImage::S8) || S.channels != 1 || T.type != S.type)
+      return false;
+   const int tw = T.width, th = T.height;
+   // (ss_box8 stages at most 127 + 97 rows)
+   if (tw <= 0 || th <= 0 || tw > 97 || th > 97) return false;
+   const long long mp = int_max_product(S.type);
+   if ((long long)tw * mp >= (1LL << 24) || (long long)tw * th * mp >= (1LL << 31)) return false;
+   return pair_geometry(kc2_for(tw), 4, th).vb >= 1 && ss_box8_lds(tw, th) <= mfma::kLdsMax;
+}
+
+size_t int_mfma_work_bytes(int tw, int th) { return pair_frag_bytes(mfma::kBF16, tw, th); }
+
+void launch_int_mfma(hipStream_t s, int op, const View &S, const View &T, double *tsums,
+                     float *dst, int dstep, void *work)
+{
+   const int tw = T.width, th = T.height, KC2 = kc2_for(tw);
+   const int src8 = S.type == SImage::U8 ? mfma::kSrcU8 : mfma::kSrcS8;
+   if (op != kCC) {
+      // the window sums of s^2 go to the map first; the MFMA kernel's epilogue reads them
+      KernelTimer kt(s, "ss_box8");
+      const int tiles_x = (S.width + kBoxCols - 1) / kBoxCols;
+      const int tiles = tiles_x * ((S.height + kBoxRows - 1) / kBoxRows);
+      const size_t lds = ss_box8_lds(tw, th);
+      if (lds > 64 * 1024)
+         hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(ss_box8),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                   "lds attribute");
+      const int v4 = dstep % 16 == 0 && reinterpret_cast<uintptr_t>(dst) % 16 == 0;
+      ss_box8<<<tiles, 256, lds, s>>>(static_cast<const unsigned char *>(S.ptr), S.width,
+                                      S.height, S.step, tw, th, src8 == mfma::kSrcS8,
+                                      reinterpret_cast<char *>(dst), dstep, tiles_x, v4);
+      launch_check("ss_box8");
+   }
+   u16x8 *g0 = static_cast<u16x8 *>(work);
+   {
+      KernelTimer kt(s, "template_prep");
+      BatchDesc d{static_cast<const char *>(T.ptr), tw, th, T.step, KC2, 1, g0, nullptr, tsums};
+      d.src8 = src8;
+      mfma::launch_pair_prep<mfma::kBF16>(s, d);
+   }
+   const Geometry g = pair_geometry(KC2, 4, th);
+   SingleLaunch L;
+   L.mode = mfma::kModeInt;
+   // 8-byte staging loads: staged columns start at 64 bx - tw/2 + 8 g, so tw/2 a multiple of 8,
+   // and 8-byte aligned rows
+   const int vec8 = (tw / 2) % 8 == 0 && S.step % 8 == 0 &&
+                    reinterpret_cast<uintptr_t>(S.ptr) % 8 == 0;
+   L.a = mfma::TileArgs{static_cast<const float *>(S.ptr), S.width, S.height, S.step, tw, th, 1,
+                        g0, nullptr, tsums, nullptr, dst, dstep, g.vb, vec8};
+   L.a.src8 = src8;
+   L.a.op = op;
+   // steps between the int32 flushes: each step adds one template row of products
+   L.a.flush = (int)(((1LL << 24) - 1) / ((long long)tw * int_max_product(S.type)));
+   L.KC = KC2;
+   L.NT = 4;
+   L.ldsp = g.ldsp;
+   L.lds = g.lds;
+   L.grid = dim3((S.width + 63) / 64, (S.height + mfma::kPairRows - 1) / mfma::kPairRows);
+   KernelTimer kt(s, "crosscorr_mfma_int");
+   mfma::launch_pair<mfma::kBF16>(s, L);
+}
+
 // ======================================================================= batch
 struct BatchEntry {
    hipGraphExec_t exec = nullptr;

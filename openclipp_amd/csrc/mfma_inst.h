@@ -62,6 +62,11 @@ template <int KC> void strip_mode(hipStream_t s, const SingleLaunch &L, int tile
 
 template <int KC> void pair_mode(hipStream_t s, const SingleLaunch &L)
 {
+   if (L.mode == kModeInt) {
+      if constexpr (P == kBF16) pair<KC, 4, kModeInt>(s, L);
+      else fail(CL_INVALID_VALUE, "exact integer sums take BF16 operands");
+      return;
+   }
    if (L.NT == 4) pair<KC, 4, kModeCC>(s, L); else pair<KC, 6, kModeCC>(s, L);
 }
 #endif

@@ -43,8 +43,10 @@ constexpr int kLdsMax = 160 * 1024 - 256;
 
 enum { kBF16 = 1, kF16 = 2, kBF16x3 = 3 };
 // kModeCC: the CrossCorr map (the single-template call; CrossCorr_Norm scales it in place
-// afterwards); kModeNCC / kModeSSDN: the batch kernel's epilogues from the FP64 integral
-enum { kModeCC = 0, kModeNCC = 1, kModeSSDN = 2 };
+// afterwards); kModeNCC / kModeSSDN: the batch kernel's epilogues from the FP64 integral;
+// kModeInt: the row-pair kernel on 8-bit integer images, exact integer sums (any of the
+// four cross-term ops, TileArgs::op)
+enum { kModeCC = 0, kModeNCC = 1, kModeSSDN = 2, kModeInt = 3 };
 
 // Arguments of one output tile computation.
 struct TileArgs {
@@ -58,6 +60,9 @@ struct TileArgs {
    int dstep;
    int vb;                       // template rows per LDS band
    int vec;                      // 16-byte staging loads allowed
+   // kModeInt (exact sums of 8-bit integer images): element type of source and template
+   // (kSrcU8 / kSrcS8), the op (ProxOp) and the steps between the int32 flushes
+   int src8 = 0, op = 0, flush = 0;
 };
 
 // One template of a batch (device table).
@@ -75,7 +80,20 @@ struct BatchDesc {
    int tw, th, tstep, KC, nub;
    u16x8 *frag0, *frag1;
    double *sums;
+   int src8 = 0;                 // template elements: F32 (0), U8 (kSrcU8), S8 (kSrcS8)
 };
+
+// 8-bit integer element types of kModeInt (their values and products are exact in BF16 and
+// FP32)
+enum { kSrcU8 = 1, kSrcS8 = 2 };
+
+// Element u of a template row as float: F32, or an 8-bit integer (exact)
+__host__ __device__ inline float tmpl_elem(const char *row, int u, int src8)
+{
+   if (src8 == kSrcU8) return (float)reinterpret_cast<const unsigned char *>(row)[u];
+   if (src8 == kSrcS8) return (float)reinterpret_cast<const signed char *>(row)[u];
+   return reinterpret_cast<const float *>(row)[u];
+}
 
 __host__ __device__ constexpr int lcols_for(int KC, int NT) { return 16 * (2 * (NT - 1) + KC); }
 
@@ -149,7 +167,7 @@ __device__ __forceinline__ void template_sqsum(const BatchDesc &d, double *red)
       for (int i = 0; i < 16; ++i) {
          const int k = k0 + 256 * i;
          const int v = k / pitch, u = k - v * pitch;
-         t[i] = (k < total && u < d.tw) ? reinterpret_cast<const float *>(d.tptr + (size_t)v * d.tstep)[u] : 0.f;
+         t[i] = (k < total && u < d.tw) ? tmpl_elem(d.tptr + (size_t)v * d.tstep, u, d.src8) : 0.f;
       }
 #pragma unroll
       for (int i = 0; i < 16; ++i) s2 += (double)t[i] * (double)t[i];
@@ -248,7 +266,8 @@ __device__ __forceinline__ void row_mfma(f32x16 (&acc)[NT], const unsigned short
 
 // Stage source rows [row0, row0 + nrows) x columns [col0, col0 + LCOLS) into LDS as 16-bit
 // values (zero outside the image), 8 columns per item, four items per thread in flight.
-template <int P, int LCOLS>
+// SRC8: the source holds 8-bit integers (a.src8 gives the sign), exact in BF16.
+template <int P, int LCOLS, bool SRC8 = false>
 __device__ __forceinline__ void stage_rows(const TileArgs &a, unsigned short *lds, int ldsp,
                                            size_t plane, int row0, int col0, int nrows)
 {
@@ -264,6 +283,25 @@ __device__ __forceinline__ void stage_rows(const TileArgs &a, unsigned short *ld
          const int r = it / groups, g = it - r * groups;
          const int sy = row0 + r, sx = col0 + 8 * g;
          const bool rowin = it < total && sy >= 0 && sy < H;
+         if constexpr (SRC8) {
+            const char *brow = reinterpret_cast<const char *>(a.src) + (size_t)(rowin ? sy : 0) * a.sstep;
+            if (a.vec && rowin && sx >= 0 && sx + 8 <= W) {
+               // a.vec: 8-byte aligned rows and staged columns starting on multiples of 8
+               const uint2 q = *reinterpret_cast<const uint2 *>(brow + sx);
+#pragma unroll
+               for (int i = 0; i < 8; ++i) {
+                  const unsigned b = ((i < 4 ? q.x : q.y) >> (8 * (i & 3))) & 0xffu;
+                  f[k][i] = a.src8 == kSrcS8 ? (float)(int)(signed char)b : (float)b;
+               }
+            } else {
+#pragma unroll
+               for (int i = 0; i < 8; ++i) {
+                  const int x = sx + i;
+                  f[k][i] = (rowin && x >= 0 && x < W) ? tmpl_elem(brow, x, a.src8) : 0.f;
+               }
+            }
+            continue;
+         }
          const float *srow = reinterpret_cast<const float *>(
              reinterpret_cast<const char *>(a.src) + (size_t)(rowin ? sy : 0) * a.sstep);
          if (a.vec && rowin && sx >= 0 && sx + 8 <= W) {
@@ -463,7 +501,8 @@ __host__ __device__ constexpr int pair_frag_steps(int th, int nub)
 // for n = lane % 32, k = 8 * (lane / 32) + i.
 template <int P>
 __device__ __forceinline__ void build_pair_fragment(const char *tptr, int tstep, int tw, int th,
-                                                    int KC, int nub, int f, int lane, u16x8 *frag)
+                                                    int KC, int nub, int f, int lane, u16x8 *frag,
+                                                    int src8)
 {
    const int S = pair_frag_steps(th, nub);
    const int kc = f % KC, s = (f / KC) % S, ub = f / (KC * S);
@@ -471,12 +510,12 @@ __device__ __forceinline__ void build_pair_fragment(const char *tptr, int tstep,
    const int v = 2 * s + 1 - dy;
    const int u0 = nub > 1 ? ub * kBlockW : 0;
    const int wb = nub > 1 ? min(kBlockW, tw - u0) : tw;
-   const float *trow = reinterpret_cast<const float *>(tptr + (size_t)(v < th ? v : 0) * tstep);
+   const char *trow = tptr + (size_t)(v < th ? v : 0) * tstep;
    u16x8 h;
 #pragma unroll
    for (int i = 0; i < 8; ++i) {
       const int u = 16 * kc + kb + i - dx;
-      const float t = (v < th && u >= 0 && u < wb) ? trow[u0 + u] : 0.f;
+      const float t = (v < th && u >= 0 && u < wb) ? tmpl_elem(trow, u0 + u, src8) : 0.f;
       unsigned short h0, h1;
       encode<P>(t, h0, h1);
       h[i] = h0;
@@ -498,7 +537,7 @@ __global__ __launch_bounds__(256) void pair_prep(BatchDesc d)
    const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
    if (f < nfrag)
       build_pair_fragment<P>(d.tptr, d.tstep, d.tw, d.th, d.KC, d.nub, f, threadIdx.x & 63,
-                             d.frag0);
+                             d.frag0, d.src8);
 }
 
 // One step: NT + KC - 1 A fragments from the LDS row `ab`, KC B fragments.
@@ -518,13 +557,25 @@ __device__ __forceinline__ void pair_mfma(f32x16 (&acc)[NT], const unsigned shor
 
 // Output tile: 127 rows x 16 NT columns (the CrossCorr map); a.vb = steps (template row pairs)
 // per LDS band.
+//
+// kModeInt (8-bit integer images, BF16 operands): every value and product is an integer that
+// BF16 and FP32 hold exactly, and so is every partial sum below 2^24.  Each step adds one
+// template row to each accumulator, at most tw max|s t| in magnitude, so after a.flush steps
+// (a.flush tw max|s t| < 2^24, set by the host) the FP32 accumulators are converted to and
+// added into int32 accumulators and cleared: the sums are exact (below 2^31, also checked by
+// the host).  The epilogue pairs the int32 halves and forms the op from the exact cross term
+// (and the window sum of squares, exact in int32, which ss_box8 wrote into the map before)
+// with the same FP64 expressions as prox_tile_int, so the maps equal that kernel's bit for bit.
 template <int P, int KC, int NT, int MODE>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(NT <= 4 ? 3 : 2, NT <= 4 ? 3 : 2)))
+__attribute__((amdgpu_waves_per_eu(NT <= 4 && MODE == kModeCC ? 3 : 2,
+                                   NT <= 4 && MODE == kModeCC ? 3 : 2)))
 void crosscorr_mfma_pair(TileArgs a, int ldsp, int tiles_x, int tiles)
 {
    static_assert(P != kBF16x3, "one operand plane");
-   static_assert(MODE == kModeCC, "the CrossCorr map");
+   static_assert(MODE == kModeCC || (MODE == kModeInt && P == kBF16 && NT == 4),
+                 "the CrossCorr map, or exact integer sums (BF16, 4 n-tiles)");
+   constexpr bool INT = MODE == kModeInt;
    constexpr int lcols = pair_lcols(KC, NT);
    constexpr int C = 16 * NT;
    constexpr int TP = pair_tile_pitch(NT);
@@ -542,14 +593,40 @@ void crosscorr_mfma_pair(TileArgs a, int ldsp, int tiles_x, int tiles)
    for (int j = 0; j < NT; ++j)
 #pragma unroll
       for (int i = 0; i < 16; ++i) acc[j][i] = 0.f;
+   // kModeInt: the int32 sums, and the steps since the last flush
+   int iacc[INT ? NT : 1][16];
+   int since = 0;
+   auto flush = [&]() {
+      if constexpr (INT) {
+#pragma unroll
+         for (int j = 0; j < NT; ++j)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+               iacc[j][i] += (int)acc[j][i];
+               acc[j][i] = 0.f;
+            }
+         since = 0;
+      }
+   };
+   auto stepped = [&]() {
+      if constexpr (INT) {
+         if (++since == a.flush) flush();
+      }
+   };
+   if constexpr (INT) {
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+#pragma unroll
+         for (int i = 0; i < 16; ++i) iacc[j][i] = 0;
+   }
 
    for (int ub = 0; ub < a.nub; ++ub) {
       for (int s0 = 0; s0 < S; s0 += a.vb) {
          const int ns = min(a.vb, S - s0);
          __syncthreads();
          // LDS row r = source row Y0 - cy + 2 s0 + r; wave w reads rows 32 w + m + 2 (s - s0)
-         stage_rows<P, lcols>(a, lds, ldsp, 0, Y0 - cy + 2 * s0, x0 - cx + ub * kBlockW,
-                              kTileRows - 2 + 2 * ns);
+         stage_rows<P, lcols, INT>(a, lds, ldsp, 0, Y0 - cy + 2 * s0, x0 - cx + ub * kBlockW,
+                                   kTileRows - 2 + 2 * ns);
          __syncthreads();
 
          // a wave whose output rows (from Y0 - 1 + 32 wave on) all lie below the image skips
@@ -565,19 +642,26 @@ void crosscorr_mfma_pair(TileArgs a, int ldsp, int tiles_x, int tiles)
             load_frags<1, KC>(a.frag0, nullptr, fb0 + (size_t)(q + 1) * KC * 64, pb, dummy);
             __builtin_amdgcn_sched_barrier(0);
             pair_mfma<P, KC, NT>(acc, arow + (size_t)(2 * q) * ldsp, pa);
+            stepped();
             const int qn = q + 2 < ns ? q + 2 : q + 1;
             load_frags<1, KC>(a.frag0, nullptr, fb0 + (size_t)qn * KC * 64, pa, dummy);
             __builtin_amdgcn_sched_barrier(0);
             pair_mfma<P, KC, NT>(acc, arow + (size_t)(2 * q + 2) * ldsp, pb);
+            stepped();
          }
-         if (q < ns) pair_mfma<P, KC, NT>(acc, arow + (size_t)(2 * q) * ldsp, pa);
+         if (q < ns) {
+            pair_mfma<P, KC, NT>(acc, arow + (size_t)(2 * q) * ldsp, pa);
+            stepped();
+         }
       }
    }
+   flush();
 
    // Pair the halves in LDS: tile row r = 32 w + m + dy holds output row Y0 - 1 + r.
    // C fragment: lane holds column n = lane % 32 of each n-tile, rows
-   // m = 8 * (i / 4) + 4 * (lane / 32) + i % 4.
+   // m = 8 * (i / 4) + 4 * (lane / 32) + i % 4.  (kModeInt: int32 halves, exact sums.)
    float *tile = reinterpret_cast<float *>(lds);
+   int *itile = reinterpret_cast<int *>(lds);
    const int n = lane & 31, dy = n >> 4, dx = n & 15;
    const int rb = 32 * wave + 4 * (lane >> 5) + dy;
    __syncthreads();
@@ -585,8 +669,10 @@ void crosscorr_mfma_pair(TileArgs a, int ldsp, int tiles_x, int tiles)
 #pragma unroll
       for (int j = 0; j < NT; ++j)
 #pragma unroll
-         for (int i = 0; i < 16; ++i)
-            tile[(rb + 8 * (i / 4) + (i % 4)) * TP + 16 * j + dx] = acc[j][i];
+         for (int i = 0; i < 16; ++i) {
+            const int at = (rb + 8 * (i / 4) + (i % 4)) * TP + 16 * j + dx;
+            if constexpr (INT) itile[at] = iacc[j][i]; else tile[at] = acc[j][i];
+         }
    }
    __syncthreads();
    if (!dy) {
@@ -594,11 +680,59 @@ void crosscorr_mfma_pair(TileArgs a, int ldsp, int tiles_x, int tiles)
       for (int j = 0; j < NT; ++j)
 #pragma unroll
          for (int i = 0; i < 16; ++i) {
-            float *p = &tile[(rb + 8 * (i / 4) + (i % 4)) * TP + 16 * j + dx];
-            *p = acc[j][i] + *p;
+            const int at = (rb + 8 * (i / 4) + (i % 4)) * TP + 16 * j + dx;
+            if constexpr (INT) itile[at] = iacc[j][i] + itile[at];
+            else tile[at] = acc[j][i] + tile[at];
          }
    }
    __syncthreads();
+   if constexpr (INT) {
+      // each output's op from its exact cross term CC (and for all but CrossCorr the window's
+      // sum of squares SS, an exact int32 that ss_box8 left in the map), as prox_tile_int forms
+      // it; four consecutive columns per thread, one 16-byte store where the row allows
+      const double tt = a.tsums[0];
+      const bool v4 = a.dstep % 16 == 0 && reinterpret_cast<uintptr_t>(a.dst) % 16 == 0;
+      for (int k = threadIdx.x; k < kPairRows * (C / 4); k += 256) {
+         const int r = k / (C / 4), c = 4 * (k - r * (C / 4));
+         const int y = Y0 + r, x = x0 + c;
+         if (y >= a.H || x >= a.W) continue;
+         const int4 cc = *reinterpret_cast<const int4 *>(&itile[(r + 1) * TP + c]);
+         const int ci[4] = {cc.x, cc.y, cc.z, cc.w};
+         float *o = reinterpret_cast<float *>(reinterpret_cast<char *>(a.dst) + (size_t)y * a.dstep) + x;
+         const int *oi = reinterpret_cast<const int *>(o);
+         int si[4] = {0, 0, 0, 0};
+         if (a.op != kCC) {
+            if (v4 && x + 4 <= a.W) {
+               const int4 q = *reinterpret_cast<const int4 *>(oi);
+               si[0] = q.x; si[1] = q.y; si[2] = q.z; si[3] = q.w;
+            } else {
+#pragma unroll
+               for (int e = 0; e < 4; ++e)
+                  if (x + e < a.W) si[e] = oi[e];
+            }
+         }
+         float v[4];
+#pragma unroll
+         for (int e = 0; e < 4; ++e) {
+            if (a.op == kCC) {
+               v[e] = (float)ci[e];
+            } else {
+               const double ss = (double)si[e];
+               const double cd = (double)ci[e];
+               const double d = (a.op == kSSD || a.op == kSSDNorm) ? ss - 2.0 * cd + tt : cd;
+               v[e] = a.op == kSSD ? (float)d : (float)(d / sqrt(ss * tt));
+            }
+         }
+         if (v4 && x + 4 <= a.W) {
+            *reinterpret_cast<float4 *>(o) = make_float4(v[0], v[1], v[2], v[3]);
+         } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+               if (x + e < a.W) o[e] = v[e];
+         }
+      }
+      return;
+   }
    // rows 1..127 are whole.  16-byte aligned output rows: four consecutive columns per
    // thread (the tile pitch keeps them 16-byte aligned in LDS too), one at a time at the
    // right edge; otherwise consecutive threads take consecutive columns

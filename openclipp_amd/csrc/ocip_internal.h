@@ -206,6 +206,14 @@ constexpr size_t kNoAllocSumsBytes = 256;   // template sums ahead of the fragme
 void noalloc_scratch_init(int device);
 void *noalloc_scratch(int device);
 
+// Exact integer sums of 8-bit images on BF16 MFMA (CrossCorr, CrossCorr_Norm, SqrDistance,
+// SqrDistance_Norm of U8/S8 1-channel images): int_mfma_ok tells whether the path takes the
+// call; tsums[0] receives sum t^2.  The maps equal prox_tile_int's bit for bit.
+bool int_mfma_ok(int op, const View &src, const View &T);
+size_t int_mfma_work_bytes(int tw, int th);
+void launch_int_mfma(hipStream_t s, int op, const View &src, const View &T, double *tsums,
+                     float *dst, int dst_step, void *work);
+
 // Batched MFMA CrossCorr_Norm (ssd = false) or SqrDistance_Norm (ssd = true) of F32 1-channel
 // templates against one F32 1-channel source, recorded into a HIP graph on first use and
 // replayed for identical arguments.  The cache belongs to the context.

@@ -296,6 +296,48 @@ def test_integer_4096_float_maps_reference_order(ocip):
         assert rel.max() <= 1e-5, op
 
 
+INT_SHAPES = [((37, 45), (1, 1)), ((70, 90), (3, 5)), ((200, 260), (32, 32)),
+              ((150, 300), (97, 40)), ((300, 130), (16, 97)), ((129, 250), (64, 64)),
+              ((260, 140), (31, 8))]
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int8])
+@pytest.mark.parametrize("shape,tshape", INT_SHAPES)
+def test_int_mfma_equals_valu_path(ocip, monkeypatch, dtype, shape, tshape):
+    """8-bit images: the exact-integer MFMA path (BF16 products, FP32 accumulators flushed
+    into int32) against the VALU kernel (prox_tile_int, OCIP_INT_MFMA=0), bit for bit, for
+    the four cross-term ops; the raw ops also against the exact int64 oracle."""
+    rng = np.random.default_rng(shape[0] + tshape[0] * 7 + np.dtype(dtype).itemsize)
+    info = np.iinfo(dtype)
+    src = rng.integers(info.min, info.max, size=shape, endpoint=True).astype(dtype)
+    tpl = rng.integers(info.min, info.max, size=tshape, endpoint=True).astype(dtype)
+    for op in ["CrossCorr", "SqrDistance", "CrossCorr_Norm", "SqrDistance_Norm"]:
+        say("int mfma", np.dtype(dtype).name, shape, tshape, op)
+        monkeypatch.delenv("OCIP_INT_MFMA", raising=False)
+        got = run_op(ocip, op, src, tpl)
+        monkeypatch.setenv("OCIP_INT_MFMA", "0")
+        want = run_op(ocip, op, src, tpl)
+        np.testing.assert_array_equal(got, want, err_msg=op)
+        if op in RAW_OPS:
+            exact = orc.f64_from_i64(OP_CODE[op], src, tpl).astype(np.float32)
+            np.testing.assert_array_equal(got, exact, err_msg=op)
+    monkeypatch.delenv("OCIP_INT_MFMA", raising=False)
+
+
+@pytest.mark.parametrize("dtype,value", [(np.uint8, 255), (np.int8, -128), (np.int8, 127)])
+def test_int_mfma_largest_products(ocip, dtype, value):
+    """Constant images and templates of the largest magnitude: every product is max|s t|, so
+    the FP32 accumulators reach the flush bound (a 97-column U8 template flushes every 2 steps)
+    and the int32 sums their largest values; against exact int64 sums."""
+    src = np.full((300, 260), value, dtype)
+    tpl = np.full((96, 97), value, dtype)
+    for op in ["CrossCorr", "SqrDistance"]:
+        say("int mfma largest", np.dtype(dtype).name, value, op)
+        got = run_op(ocip, op, src, tpl)
+        exact = orc.f64_from_i64(OP_CODE[op], src, tpl).astype(np.float32)
+        np.testing.assert_array_equal(got, exact, err_msg=op)
+
+
 def test_precision_4096_plain_bands(ocip):
     from openclipp_amd import workloads
     src, tpl, (ox, oy) = workloads.precision_4096()
