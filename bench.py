@@ -149,8 +149,8 @@ def configs_bench(ocip, steps):
     the 1024^2 F32 / 16x16 plain path (SqrDistance and CrossCorr_Norm; for a template this
     small the kernel is bounded by HBM, so its achieved GB/s are given with the algorithmic
     bytes source + map + template), the 4096^2 U8 / 32x32 integer workload (all five ops,
-    exact integer sums), and the gradients of the 16 x 1024^2 / 31x31 workload (both backward
-    calls per image, 16 images per step)."""
+    exact integer sums), and the gradients of the 16 x 1024^2 / 31x31 workload (one
+    ocipProximityBackwardBatch call per step: both ops' gradients for the 16 images)."""
     from openclipp_amd import workloads
     out = {}
 
@@ -212,16 +212,13 @@ def configs_bench(ocip, steps):
     Ss = [ocip.Image(srcs[k]).send() for k in range(n)]
     G1 = [ocip.Image(w_ncc[k]).send() for k in range(n)]
     G2 = [ocip.Image(w_ssd[k]).send() for k in range(n)]
-    dS = ocip.Image(shape=(H, W), dtype=np.float32)
+    dSs = [ocip.Image(shape=(H, W), dtype=np.float32, device_only=True) for _ in range(n)]
     dT = ocip.Image(shape=(th, tw), dtype=np.float32)
+    hS, h1, h2, hdS = ocip.handles(Ss), ocip.handles(G1), ocip.handles(G2), ocip.handles(dSs)
 
     def grads():
-        for k in range(n):
-            ocip.check(ocip.lib.ocipCrossCorr_NormBackward(Ss[k].handle, T.handle, G1[k].handle,
-                                                            dS.handle, dT.handle))
-            ocip.check(ocip.lib.ocipSqrDistanceBackward(Ss[k].handle, T.handle, G2[k].handle,
-                                                         dS.handle, dT.handle))
-        return ocip.CL_SUCCESS
+        # one call: both ops' gradients for the 16 images, dT summed over the batch
+        return ocip.lib.ocipProximityBackwardBatch(hS, T.handle, h1, h2, hdS, dT.handle, n)
 
     ms, kms = time_steps(ocip, grads, max(2, steps // 8), 1, "backward")
     # algorithmic work: 2 FLOP per tpix per gradient map, two maps (source and template) per
@@ -229,14 +226,15 @@ def configs_bench(ocip, steps):
     gflop = 2.0 * 2 * 2 * n * W * H * tw * th / 1e9
     out["diff_1024x16"] = {
         "workload": f"{n} x {W}x{H} F32 U[0,1), {tw}x{th} N(0,1) template: CrossCorr_Norm and "
-                    f"SqrDistance gradients wrt images and template, {n} images per step",
+                    f"SqrDistance gradients wrt images and template, {n} images per step "
+                    f"(ocipProximityBackwardBatch)",
         "ms_per_step": round(ms, 4),
         "gradient_gpix_tpix_per_s": round(2 * 2 * n * W * H * tw * th / (ms * 1e-3) / 1e9, 1),
         "roofline": {"bound": "valu", "achieved": round(gflop / ms, 2),
                      "peak": PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
                      "frac": round(gflop / ms / PEAK_TFLOPS["f32"], 4),
-                     "note": "whole step (every kernel of the 32 backward calls)"}}
-    for i in Ss + G1 + G2 + [T, dS, dT]:
+                     "note": "whole step (every kernel of the batched backward call)"}}
+    for i in Ss + G1 + G2 + dSs + [T, dT]:
         i.release()
     return out
 

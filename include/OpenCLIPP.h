@@ -217,6 +217,21 @@ ocipError ocip_API ocipSqrDistanceBackward(ocipImage Source, ocipImage Template,
                                            ocipImage GradDest, ocipImage GradSource,
                                            ocipImage GradTemplate);
 
+/* Gradients over a batch of same-size F32 1-channel sources sharing one template, of
+ *   L = sum_i [ sum(GradNcc[i] * CrossCorr_Norm(Sources[i], Template))
+ *             + sum(GradSsd[i] * SqrDistance(Sources[i], Template)) ]
+ * GradSources[i] (Source-sized) receives dL/dSources[i]; GradTemplate (Template-sized) receives
+ * dL/dTemplate, summed over the batch.  GradNcc or GradSsd may be NULL (that term absent);
+ * GradSources, its entries, and GradTemplate may be NULL (that gradient skipped).  The two
+ * ops' gradients share their correlations (one map per image combines both).  Equivalent to
+ * Count calls of ocipCrossCorr_NormBackward and ocipSqrDistanceBackward with the results
+ * added. */
+ocipError ocip_API ocipProximityBackwardBatch(const ocipImage * Sources, ocipImage Template,
+                                              const ocipImage * GradNcc,
+                                              const ocipImage * GradSsd,
+                                              ocipImage * GradSources, ocipImage GradTemplate,
+                                              uint Count);
+
 /* Kernel timing (profiling aid).  While enabled, every kernel the library launches is
  * bracketed by HIP events on the stream it runs on.  ocipGetKernelTime waits for the queued
  * work and returns the total time (ms) and launch count of the kernels whose name starts

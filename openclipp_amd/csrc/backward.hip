@@ -3,7 +3,7 @@
 // The reference has no gradients; the forward maps are cl-files/ImageProximity.cl:44-73
 // (SqrDistance) and :375-412 (CrossCorr_Norm).  With the template origin at (-cx, -cy),
 // P the zero-padded source, G the gradient of the loss with respect to the output map,
-// C = sum P t, SS = window sum of P^2 (from the FP64 integral), TT = sum t^2,
+// C = sum P t, SS = window sum of P^2 (in FP64), TT = sum t^2,
 // D = sqrt(SS TT) and N = C / D:
 //
 //   CrossCorr_Norm   dL/dt(u,v) = sum_{x,y} (G/D)(x,y) P(x+u-cx, y+v-cy) - (sum G N) t(u,v) / TT
@@ -15,10 +15,14 @@
 // (map values outside the image are 0; outputs whose denominator D is 0 -- NaN in the
 // forward map -- contribute nothing.)
 //
-// Building blocks: weight_corr (sum over all output pixels of a map times the shifted
-// source, one value per template tap, FP32 partials per tile reduced in FP64), the FP32
+// Building blocks: the template-gradient contraction (sum over all output pixels of a map
+// times the shifted source, one value per template tap: tgrad_mfma on split-BF16 MFMA for
+// templates up to 32 x 32, weight_corr in FP32 otherwise; partials reduced in FP64), the FP32
 // correlation of a map with the flipped template (the plain tiled kernel with anchor
-// (tw-1-cx, th-1-cy)), and flipped window sums from an FP64 integral of a map.
+// (tw-1-cx, th-1-cy)), and window sums of maps in FP64 (box_f64; from an FP64 integral for
+// templates too large for its tile).
+#include <algorithm>
+
 #include "device_common.h"
 
 namespace ocip {
@@ -37,53 +41,151 @@ __device__ __forceinline__ double block_sum_d(double v, double *red)
    return t;
 }
 
-// Per-pixel maps of the CrossCorr_Norm gradient: Gd = G / D, E = G N / SS (0 where D = 0),
-// and per-block partial sums of G N.
-__global__ __launch_bounds__(kThreads) void ncc_maps(const float *C, int cstep, const char *G,
-                                                      int gstep, const double *__restrict__ integ,
-                                                      const double *__restrict__ tsums, int W,
-                                                      int H, int tw, int th, float *Gd, float *E,
-                                                      double *partial)
+// Window sums of a map (or of its squares) in FP64: out[y * W + x] = sum over v < th, u < tw
+// of f(m(x + u - ax, y + v - ay)), f = identity or square, zero outside the image.  One
+// workgroup per 32 x 64 outputs: the map region (31 + th rows by 63 + tw columns) is staged in
+// LDS as F32; a vertical pass (thread = column) keeps two FP64 running sums down the rows from
+// the region's top, `lead` taking each row as it enters the window and `lag` each row as it
+// leaves, so the column sum V = lead - lag is a difference of two prefixes of one sequence
+// (exactly 0 for a column whose window rows are all zero); V goes to LDS in FP64; a horizontal
+// pass (thread = output row and 8 columns) does the same along the row.  Every value of f is
+// exact in FP64 (a float or the square of a float), and the prefixes start at the tile, so the
+// relative error stays near 2^-53 times (tile energy / window energy).
+constexpr int kBxRows = 32, kBxCols = 64;
+
+size_t box_f64_lds(int tw, int th)
+{
+   const size_t ncols = kBxCols - 1 + tw, nrows = kBxRows - 1 + th;
+   return (size_t)kBxRows * ncols * sizeof(double) + nrows * ncols * sizeof(float);
+}
+
+__global__ __launch_bounds__(kThreads) void box_f64(const char *m, int mstep, int W, int H,
+                                                     int tw, int th, int ax, int ay, int sqr,
+                                                     double *out, int tiles_x)
+{
+   extern __shared__ double bx_lds[];
+   const int ncols = kBxCols - 1 + tw, nrows = kBxRows - 1 + th;
+   double *vs = bx_lds;                                            // kBxRows x ncols
+   float *ms = reinterpret_cast<float *>(vs + kBxRows * ncols);   // nrows x ncols
+   const int x0 = (blockIdx.x % tiles_x) * kBxCols, y0 = (blockIdx.x / tiles_x) * kBxRows;
+   const int c0 = x0 - ax, r0 = y0 - ay;
+   const int total = nrows * ncols;
+   // staging: eight loads of each thread in flight at a time
+   for (int k0 = threadIdx.x; k0 < total; k0 += 8 * kThreads) {
+      float f[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+         const int k = k0 + j * kThreads;
+         const int r = k / ncols, c = k - r * ncols;
+         const int y = r0 + r, x = c0 + c;
+         f[j] = (k < total && y >= 0 && y < H && x >= 0 && x < W)
+                    ? reinterpret_cast<const float *>(m + (size_t)y * mstep)[x]
+                    : 0.f;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+         const int k = k0 + j * kThreads;
+         if (k < total) ms[k] = f[j];
+      }
+   }
+   __syncthreads();
+   auto fv = [&](int r, int c) {
+      const double v = ms[r * ncols + c];
+      return sqr ? v * v : v;
+   };
+   for (int c = threadIdx.x; c < ncols; c += kThreads) {
+      double lead = 0.0, lag = 0.0;
+      for (int i = 0; i < th - 1; ++i) lead += fv(i, c);
+      for (int r = 0; r < kBxRows; ++r) {
+         lead += fv(r + th - 1, c);
+         vs[r * ncols + c] = lead - lag;
+         lag += fv(r, c);
+      }
+   }
+   __syncthreads();
+   const int r = threadIdx.x >> 3, q = (threadIdx.x & 7) * 8;
+   const int y = y0 + r;
+   if (y >= H) return;
+   const double *vrow = vs + r * ncols + q;
+   double lead = 0.0, lag = 0.0;
+   for (int k = 0; k < tw - 1; ++k) lead += vrow[k];
+   double *orow = out + (size_t)y * W;
+#pragma unroll
+   for (int j = 0; j < 8; ++j) {
+      lead += vrow[j + tw - 1];
+      if (x0 + q + j < W) orow[x0 + q + j] = lead - lag;
+      lag += vrow[j];
+   }
+}
+
+void launch_box_f64(hipStream_t s, const char *m, int mstep, int W, int H, int tw, int th,
+                    int ax, int ay, bool sqr, double *out)
+{
+   const int tiles_x = (W + kBxCols - 1) / kBxCols;
+   const int tiles = tiles_x * ((H + kBxRows - 1) / kBxRows);
+   const size_t lds = box_f64_lds(tw, th);
+   if (lds > 64 * 1024)
+      hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(box_f64),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                "lds attribute");
+   box_f64<<<tiles, kThreads, lds, s>>>(m, mstep, W, H, tw, th, ax, ay, sqr ? 1 : 0, out, tiles_x);
+   launch_check("box_f64");
+}
+
+// Per-pixel maps of the gradient of L = sum G1 NCC + sum G2 SSD (G1 or G2 null: that term
+// absent), with D = sqrt(SS TT) and N = C / D (the first terms 0 where D = 0):
+//   M  = G1 / D - 2 G2          (dT: its correlation with the source; dS: its correlation
+//                                with the flipped template)
+//   Eb = 2 G2 - G1 N / SS       (dS: s times its flipped window sums)
+// each formed in FP64 and rounded once, and per-block partial sums of G1 N (sncc) and of G2
+// (sssd).  SS comes from the dense window-sum map `box` (box_f64), or when that is null from
+// the exclusive FP64 integral `integ`.
+__global__ __launch_bounds__(kThreads) void grad_maps(const float *C, int cstep, const char *G1,
+                                                       int g1step, const char *G2, int g2step,
+                                                       const double *__restrict__ box,
+                                                       const double *__restrict__ integ,
+                                                       const double *__restrict__ tsums, int W,
+                                                       int H, int tw, int th, float *M, float *Eb,
+                                                       double *sncc, double *sssd)
 {
    __shared__ double red[4];
    const double tt = tsums[0];
    const int cx = tw / 2, cy = th / 2;
-   double acc = 0;
+   double a1 = 0, a2 = 0;
    for (long long k = (long long)blockIdx.x * kThreads + threadIdx.x; k < (long long)W * H;
         k += (long long)gridDim.x * kThreads) {
       const int y = (int)(k / W), x = (int)(k - (long long)y * W);
-      const double ss = rect_sum(integ, W, H, x - cx, x - cx + tw, y - cy, y - cy + th);
-      const double d = sqrt(ss * tt);
-      const double g = reinterpret_cast<const float *>(G + (size_t)y * gstep)[x];
-      const double c = reinterpret_cast<const float *>(reinterpret_cast<const char *>(C) +
-                                                       (size_t)y * cstep)[x];
-      float gd = 0.f, e = 0.f;
-      if (d > 0) {
-         const double n = c / d;
-         gd = (float)(g / d);
-         e = (float)(g * n / ss);
-         acc += g * n;
+      double m = 0.0, e = 0.0;
+      if (G1) {
+         const double ss = box ? box[k] : rect_sum(integ, W, H, x - cx, x - cx + tw, y - cy, y - cy + th);
+         const double d = sqrt(ss * tt);
+         const double g = reinterpret_cast<const float *>(G1 + (size_t)y * g1step)[x];
+         const double c = reinterpret_cast<const float *>(reinterpret_cast<const char *>(C) +
+                                                          (size_t)y * cstep)[x];
+         if (d > 0) {
+            const double nn = c / d;
+            m = g / d;
+            e = -(g * nn / ss);
+            a1 += g * nn;
+         }
       }
-      Gd[k] = gd;
-      E[k] = e;
+      if (G2) {
+         const double g = reinterpret_cast<const float *>(G2 + (size_t)y * g2step)[x];
+         m -= 2.0 * g;
+         e += 2.0 * g;
+         a2 += g;
+      }
+      M[k] = (float)m;
+      Eb[k] = (float)e;
    }
-   acc = block_sum_d(acc, red);
-   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
-}
-
-// Per-block partial sums of a pitched F32 map.
-__global__ __launch_bounds__(kThreads) void map_sum(const char *G, int gstep, int W, int H,
-                                                     double *partial)
-{
-   __shared__ double red[4];
-   double acc = 0;
-   for (long long k = (long long)blockIdx.x * kThreads + threadIdx.x; k < (long long)W * H;
-        k += (long long)gridDim.x * kThreads) {
-      const int y = (int)(k / W), x = (int)(k - (long long)y * W);
-      acc += reinterpret_cast<const float *>(G + (size_t)y * gstep)[x];
+   if (G1) {
+      a1 = block_sum_d(a1, red);
+      if (threadIdx.x == 0) sncc[blockIdx.x] = a1;
    }
-   acc = block_sum_d(acc, red);
-   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+   if (G2) {
+      a2 = block_sum_d(a2, red);
+      if (threadIdx.x == 0) sssd[blockIdx.x] = a2;
+   }
 }
 
 // partial[b][v * tw + u] = sum over the tile's output pixels of M(x, y) * P(x+u-cx, y+v-cy)
@@ -150,22 +252,169 @@ __global__ __launch_bounds__(kThreads) void weight_corr(const char *M, int mstep
    }
 }
 
-// dT(u, v) = a * sum_b partial[b][tap] + b * t(u, v) with (a, b) from the scalar sums:
-//   mode 0 (CrossCorr_Norm): a = 1,  b = -(sum G N) / TT
-//   mode 1 (SqrDistance):    a = -2, b = 2 sum G
+// ---- the template-gradient contraction on MFMA (templates up to 32 x 32) -------------------
+// partial[(task, wave)][v * tw + u] = sum over the task's pixels (y, x') of
+//    A[v][(y, x')] * B[(y, x')][u],   A[v][(y, x')] = P(x', y + v - cy),  B[(y, x')][u] = M(x' - u + cx, y)
+// i.e. sum over x, y of M(x, y) P(x + u - cx, y + v - cy) with x' = x + u - cx: a GEMM with
+// M = 32 rows (template rows v), N = 32 columns (template columns u) and K = the image's pixels,
+// on v_mfma_f32_32x32x16_bf16 with split BF16 operands (x = hi + lo, three products hi hi,
+// hi lo, lo hi: relative error below 2^-14 per product) and FP32 accumulation.  Lane l holds
+// A[m][k] and B[k][n] for m, n = l % 32 and k = 8 (l / 32) + i (i < 8), and C[m][n] for
+// m = 8 (i / 4) + 4 (l / 32) + i % 4 (the layouts the forward MFMA kernels use).
+// A task is kTgRows rows y by kTgCols columns x' of the image: the source rows it reads
+// (y - cy .. y - cy + 31) and the map rows (columns x' - 31 + cx .. x' + cx) are staged in LDS
+// as BF16 hi and lo planes, the map twice (the second copy one element to the left), so that
+// every B read of 8 consecutive elements starts on a 4-byte boundary in one of the copies.
+// Wave w takes the task's rows y0 + w, y0 + w + 4, ...; the four waves' 32 x 32 sums are
+// added in wave order and the task writes one partial per tap, which grad_template reduces in
+// FP64.
+constexpr int kTgRows = 16, kTgCols = 128;
+constexpr int kTgPRows = kTgRows + 31;       // staged source rows
+constexpr int kTgMCols = kTgCols + 32;       // staged map columns (31 to the left, 1 spare)
+
+typedef __bf16 tg_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float tg_f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned short tg_u16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ unsigned short tg_hi(float f)
+{
+   return __builtin_bit_cast(unsigned short, (__bf16)f);
+}
+__device__ __forceinline__ unsigned short tg_lo(float f, unsigned short hi)
+{
+   return __builtin_bit_cast(unsigned short,
+                             (__bf16)(f - (float)__builtin_bit_cast(__bf16, hi)));
+}
+__device__ __forceinline__ tg_f32x16 tg_mma(tg_u16x8 a, tg_u16x8 b, tg_f32x16 c)
+{
+   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(tg_bf16x8, a),
+                                                  __builtin_bit_cast(tg_bf16x8, b), c, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(256) void tgrad_mfma(const char *M, int mstep, const float *S,
+                                                   int sstep, int W, int H, int tw, int th,
+                                                   int tasks_x, float *partial)
+{
+   // planes: source hi, source lo (kTgPRows x kTgCols); map hi, lo, each in two copies
+   // (kTgRows x kTgMCols)
+   __shared__ unsigned short ps[2][kTgPRows * kTgCols];
+   __shared__ unsigned short ms[2][2][kTgRows * kTgMCols];
+   const int cx = tw / 2, cy = th / 2;
+   const int tx = blockIdx.x % tasks_x, ty = blockIdx.x / tasks_x;
+   const int xb = tx * kTgCols, y0 = ty * kTgRows;
+   // source rows y0 - cy .. y0 - cy + kTgPRows - 1, columns xb .. xb + kTgCols - 1 (0 outside),
+   // and map rows y0 .. y0 + kTgRows - 1 (copy 0 element j = map column xb + cx - 31 + j,
+   // copy 1 element j = copy 0 element j + 1): every load of the thread issued before any
+   // store (the loop counts are constants)
+   constexpr int kPIt = (kTgPRows * kTgCols + 255) / 256, kMIt = (kTgRows * kTgMCols + 255) / 256;
+   float pf[kPIt], mf0[kMIt], mf1[kMIt];
+#pragma unroll
+   for (int it = 0; it < kPIt; ++it) {
+      const int k = threadIdx.x + 256 * it;
+      const int r = k / kTgCols, c = k - r * kTgCols;
+      const int y = y0 - cy + r, x = xb + c;
+      pf[it] = (k < kTgPRows * kTgCols && y >= 0 && y < H && x < W)
+                   ? reinterpret_cast<const float *>(reinterpret_cast<const char *>(S) +
+                                                     (size_t)y * sstep)[x]
+                   : 0.f;
+   }
+#pragma unroll
+   for (int it = 0; it < kMIt; ++it) {
+      const int k = threadIdx.x + 256 * it;
+      const int r = k / kTgMCols, j = k - r * kTgMCols;
+      const int y = y0 + r;
+      const bool rowin = k < kTgRows * kTgMCols && y < H;
+      const float *mrow = reinterpret_cast<const float *>(M + (size_t)(rowin ? y : 0) * mstep);
+      const int x0c = xb + cx - 31 + j;
+      mf0[it] = (rowin && x0c >= 0 && x0c < W) ? mrow[x0c] : 0.f;
+      mf1[it] = (rowin && x0c + 1 >= 0 && x0c + 1 < W) ? mrow[x0c + 1] : 0.f;
+   }
+#pragma unroll
+   for (int it = 0; it < kPIt; ++it) {
+      const int k = threadIdx.x + 256 * it;
+      if (k >= kTgPRows * kTgCols) break;
+      const unsigned short h = tg_hi(pf[it]);
+      ps[0][k] = h;
+      ps[1][k] = tg_lo(pf[it], h);
+   }
+#pragma unroll
+   for (int it = 0; it < kMIt; ++it) {
+      const int k = threadIdx.x + 256 * it;
+      if (k >= kTgRows * kTgMCols) break;
+      const unsigned short h0 = tg_hi(mf0[it]), h1 = tg_hi(mf1[it]);
+      ms[0][0][k] = h0;
+      ms[1][0][k] = tg_lo(mf0[it], h0);
+      ms[0][1][k] = h1;
+      ms[1][1][k] = tg_lo(mf1[it], h1);
+   }
+   __syncthreads();
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   const int mn = lane & 31, g = lane >> 5;
+   tg_f32x16 acc;
+#pragma unroll
+   for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+   for (int r = wave; r < kTgRows; r += 4) {
+      if (y0 + r >= H) break;
+      // A: source row (task row r + m), columns 16 kb + 8 g .. + 7
+      const unsigned short *ah = &ps[0][(r + mn) * kTgCols + 8 * g];
+      const unsigned short *al = &ps[1][(r + mn) * kTgCols + 8 * g];
+#pragma unroll 2
+      for (int kb = 0; kb < kTgCols / 16; ++kb) {
+         if (xb + 16 * kb >= W) break;
+         const tg_u16x8 a_hi = *reinterpret_cast<const tg_u16x8 *>(ah + 16 * kb);
+         const tg_u16x8 a_lo = *reinterpret_cast<const tg_u16x8 *>(al + 16 * kb);
+         // B: map elements at copy-0 index 16 kb + 8 g - n + 31 .. + 7 (n = mn)
+         const int s0 = 16 * kb + 8 * g - mn + 31;
+         const int cp = s0 & 1, base = r * kTgMCols + (s0 - cp);
+         tg_u16x8 b_hi, b_lo;
+#pragma unroll
+         for (int q = 0; q < 4; ++q) {
+            const unsigned wh = *reinterpret_cast<const unsigned *>(&ms[0][cp][base + 2 * q]);
+            const unsigned wl = *reinterpret_cast<const unsigned *>(&ms[1][cp][base + 2 * q]);
+            b_hi[2 * q] = (unsigned short)(wh & 0xffffu);
+            b_hi[2 * q + 1] = (unsigned short)(wh >> 16);
+            b_lo[2 * q] = (unsigned short)(wl & 0xffffu);
+            b_lo[2 * q + 1] = (unsigned short)(wl >> 16);
+         }
+         acc = tg_mma(a_hi, b_hi, acc);
+         acc = tg_mma(a_hi, b_lo, acc);
+         acc = tg_mma(a_lo, b_hi, acc);
+      }
+   }
+   // the four waves' C[m][n] summed in wave order through LDS (the source planes are no longer
+   // read), then partial[task][m * tw + n] for m < th, n < tw
+   __syncthreads();
+   float *red = reinterpret_cast<float *>(&ps[0][0]);   // 4 x 32 x 32 floats
+#pragma unroll
+   for (int i = 0; i < 16; ++i) {
+      const int m = 8 * (i / 4) + 4 * g + i % 4;
+      red[(wave * 32 + m) * 32 + mn] = acc[i];
+   }
+   __syncthreads();
+   float *out = partial + (size_t)blockIdx.x * (size_t)(tw * th);
+   for (int k = threadIdx.x; k < 32 * 32; k += 256) {
+      const int m = k >> 5, n = k & 31;
+      if (m < th && n < tw)
+         out[m * tw + n] = ((red[k] + red[1024 + k]) + red[2048 + k]) + red[3072 + k];
+   }
+}
+
+// dT(u, v) = sum_b partial[b][tap] + c t(u, v), c = 2 (sum G2) - (sum G1 N) / TT, the sums
+// from the per-block partials sssd (nssd of them) and sncc (nncc; either count may be 0).
 __global__ __launch_bounds__(kThreads) void grad_template(const float *partial, int nblocks,
-                                                           const double *scal_partial,
-                                                           int nscal, const float *tstage,
-                                                           int tpitch, const double *tsums,
-                                                           int tw, int th, int mode, char *dT,
-                                                           int dtstep)
+                                                           const double *sncc, int nncc,
+                                                           const double *sssd, int nssd,
+                                                           const float *tstage, int tpitch,
+                                                           const double *tsums, int tw, int th,
+                                                           char *dT, int dtstep)
 {
    __shared__ double red[4];
-   double sc = 0;
-   for (int k = threadIdx.x; k < nscal; k += kThreads) sc += scal_partial[k];
-   sc = block_sum_d(sc, red);
-   const double a = mode == 0 ? 1.0 : -2.0;
-   const double b = mode == 0 ? -sc / tsums[0] : 2.0 * sc;
+   double s1 = 0, s2 = 0;
+   for (int k = threadIdx.x; k < nncc; k += kThreads) s1 += sncc[k];
+   for (int k = threadIdx.x; k < nssd; k += kThreads) s2 += sssd[k];
+   s1 = block_sum_d(s1, red);
+   s2 = block_sum_d(s2, red);
+   const double cterm = 2.0 * s2 - s1 / tsums[0];
    // 64 taps per workgroup (lane = tap); the four waves sum the blocks k = wave (mod 4),
    // then lane's four sums are added in wave order
    __shared__ double part[4][64];
@@ -173,15 +422,25 @@ __global__ __launch_bounds__(kThreads) void grad_template(const float *partial, 
    for (int t0 = blockIdx.x * 64; t0 < ntaps; t0 += gridDim.x * 64) {
       const int tap = t0 + lane;
       double r = 0;
-      if (tap < ntaps)
-         for (int k = wave; k < nblocks; k += 4) r += partial[(size_t)k * ntaps + tap];
+      if (tap < ntaps) {
+         // blocks k = wave (mod 4): eight at a time (loads in flight), added in order
+         int k = wave;
+         for (; k + 28 < nblocks; k += 32) {
+            float q[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) q[j] = partial[(size_t)(k + 4 * j) * ntaps + tap];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) r += q[j];
+         }
+         for (; k < nblocks; k += 4) r += partial[(size_t)k * ntaps + tap];
+      }
       part[wave][lane] = r;
       __syncthreads();
       if (wave == 0 && tap < ntaps) {
          r = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
          const int v = tap / tw, u = tap - v * tw;
          const double t = tstage[(size_t)v * tpitch + u];
-         reinterpret_cast<float *>(dT + (size_t)v * dtstep)[u] = (float)(a * r + b * t);
+         reinterpret_cast<float *>(dT + (size_t)v * dtstep)[u] = (float)(r + cterm * t);
       }
       __syncthreads();
    }
@@ -196,45 +455,60 @@ __global__ void flip_template(const float *tstage, int tpitch, int tw, int th, f
    }
 }
 
-// dS(j, i) = alpha * F(j, i) + beta * s(j, i) * (flipped window sum of the map integrated in I)
+// dS(j, i) = F(j, i) + s(j, i) * (flipped window sum of a map: the dense window-sum map `bm`
+// (box_f64), or when that is null from the map's integral I), in FP64, rounded once
 __global__ __launch_bounds__(kThreads) void grad_source(const float *F, const float *S, int sstep,
+                                                         const double *__restrict__ bm,
                                                          const double *__restrict__ I, int W,
-                                                         int H, int tw, int th, float alpha,
-                                                         float beta, char *dS, int dsstep)
+                                                         int H, int tw, int th, char *dS,
+                                                         int dsstep)
 {
    const int cx = tw / 2, cy = th / 2;
    for (long long k = (long long)blockIdx.x * kThreads + threadIdx.x; k < (long long)W * H;
         k += (long long)gridDim.x * kThreads) {
       const int i = (int)(k / W), j = (int)(k - (long long)i * W);
-      const double box = rect_sum(I, W, H, j + cx - tw + 1, j + cx + 1, i + cy - th + 1, i + cy + 1);
+      const double box = bm ? bm[k] : rect_sum(I, W, H, j + cx - tw + 1, j + cx + 1, i + cy - th + 1, i + cy + 1);
       const double s = reinterpret_cast<const float *>(reinterpret_cast<const char *>(S) +
                                                        (size_t)i * sstep)[j];
-      reinterpret_cast<float *>(dS + (size_t)i * dsstep)[j] =
-          (float)(alpha * (double)F[k] + beta * s * box);
+      reinterpret_cast<float *>(dS + (size_t)i * dsstep)[j] = (float)((double)F[k] + s * box);
    }
 }
 
 int grid_for(long long n) { return (int)std::min<long long>(1024, (n + kThreads - 1) / kThreads); }
 
 struct BwLayout {
-   size_t F, Gd, E, I, partial, scal, tflip, total;
+   size_t F, M, E, I, partial, scal, tflip, total;
+   int nb;   // template-gradient partial blocks per image
 };
 
 size_t align_up(size_t n) { return (n + 255) / 256 * 256; }
 
-BwLayout bw_layout(int W, int H, int tw, int th, int tpitch)
+// the template-gradient contraction: tgrad_mfma for templates up to 32 x 32, else weight_corr
+// on 64- (or, when its LDS does not fit, 32-) pixel tiles
+bool tgrad_on_mfma(int tw, int th) { return tw <= 32 && th <= 32; }
+int wc_tile(int tw, int th)
+{
+   return (64 + th - 1) * (64 + tw - 1 + 4) * 4 + 64 * 64 * 4 <= 64 * 1024 ? 64 : 32;
+}
+int tgrad_blocks(int W, int H, int tw, int th)
+{
+   if (tgrad_on_mfma(tw, th)) return ((W + kTgCols - 1) / kTgCols) * ((H + kTgRows - 1) / kTgRows);
+   const int t = wc_tile(tw, th);
+   return ((W + t - 1) / t) * ((H + t - 1) / t);
+}
+
+BwLayout bw_layout(int n, int W, int H, int tw, int th, int tpitch)
 {
    const size_t px = (size_t)W * H;
-   // weight_corr partials: room for 32-pixel tiles (at most 4x the 64-pixel tile count)
-   const size_t nb = (size_t)((W + 31) / 32) * ((H + 31) / 32);
    BwLayout L;
+   L.nb = tgrad_blocks(W, H, tw, th);
    size_t o = 0;
    L.F = o;       o = align_up(o + px * sizeof(float));
-   L.Gd = o;      o = align_up(o + px * sizeof(float));
+   L.M = o;       o = align_up(o + px * sizeof(float));
    L.E = o;       o = align_up(o + px * sizeof(float));
-   L.I = o;       o = align_up(o + excl_bytes(W, H));
-   L.partial = o; o = align_up(o + nb * tw * th * sizeof(float));
-   L.scal = o;    o = align_up(o + 1024 * sizeof(double));
+   L.I = o;       o = align_up(o + excl_bytes(W, H));    // a dense W x H FP64 map fits too
+   L.partial = o; o = align_up(o + (size_t)n * L.nb * tw * th * sizeof(float));
+   L.scal = o;    o = align_up(o + (size_t)2 * n * 1024 * sizeof(double));
    L.tflip = o;   o = align_up(o + (size_t)tpitch * th * sizeof(float));
    L.total = o;
    return L;
@@ -242,88 +516,112 @@ BwLayout bw_layout(int W, int H, int tw, int th, int tpitch)
 
 }  // namespace
 
-size_t backward_work_bytes(int W, int H, int tw, int th, int tpitch)
+size_t backward_work_bytes(int n, int W, int H, int tw, int th, int tpitch)
 {
-   return bw_layout(W, H, tw, th, tpitch).total;
+   return bw_layout(n, W, H, tw, th, tpitch).total;
 }
 
-// mode 0: CrossCorr_Norm, 1: SqrDistance.  integ_ss: exclusive FP64 integral of s^2 (mode 0).
-void launch_backward(hipStream_t s, int mode, const View &S, const View &G, const float *tstage,
-                     int tpitch, int tw, int th, const double *tsums, const double *integ_ss,
-                     void *work, void *aux, char *dS, int dsstep, char *dT, int dtstep)
+// Gradients of L = sum_i [sum G1_i NCC(S_i, t) + sum G2_i SSD(S_i, t)] (G1 / G2 null: that op
+// absent): dS_i for the non-null dS[i], dT (if not null) summed over the batch.
+void launch_backward_batch(hipStream_t s, int n, const View *S, const View *G1, const View *G2,
+                           const float *tstage, int tpitch, int tw, int th, const double *tsums,
+                           void *work, void *aux, char *const *dS, const int *dsstep, char *dT,
+                           int dtstep)
 {
-   const int W = S.width, H = S.height;
+   const int W = S[0].width, H = S[0].height;
    const size_t px = (size_t)W * H;
-   const BwLayout L = bw_layout(W, H, tw, th, tpitch);
+   const BwLayout L = bw_layout(n, W, H, tw, th, tpitch);
    char *w = static_cast<char *>(work);
    float *F = reinterpret_cast<float *>(w + L.F);
-   float *Gd = reinterpret_cast<float *>(w + L.Gd);
+   float *M = reinterpret_cast<float *>(w + L.M);
    float *E = reinterpret_cast<float *>(w + L.E);
    double *I = reinterpret_cast<double *>(w + L.I);
    float *partial = reinterpret_cast<float *>(w + L.partial);
-   double *scal = reinterpret_cast<double *>(w + L.scal);
+   double *sncc = reinterpret_cast<double *>(w + L.scal);
+   double *sssd = sncc + (size_t)n * 1024;
    float *tflip = reinterpret_cast<float *>(w + L.tflip);
-   const int tile = (64 + th - 1) * (64 + tw - 1 + 4) * 4 + 64 * 64 * 4 <= 64 * 1024 ? 64 : 32;
-   const int nbx = (W + tile - 1) / tile, nby = (H + tile - 1) / tile;
    const int nscal = grid_for((long long)px);
-   const float *Sp = static_cast<const float *>(S.ptr);
+   const int taps = tw * th;
+   // window sums straight from the maps (box_f64) when its LDS tile fits, else from integrals
+   const bool use_box = box_f64_lds(tw, th) <= (size_t)(160 * 1024 - 256);
    KernelTimer kt(s, "backward");
 
-   // the map M that multiplies the shifted source in dT, and that is correlated with the
-   // flipped template in dS
-   const char *M;
-   int mstep;
-   if (mode == 0) {
-      // forward CC with the same order as ocipCrossCorr, then the per-pixel maps
-      launch_crosscorr_f32_anchored(s, S, tstage, tpitch, tw, th, tw / 2, th / 2, F, W * 4);
-      ncc_maps<<<nscal, kThreads, 0, s>>>(F, W * 4, static_cast<const char *>(G.ptr), G.step,
-                                           integ_ss, tsums, W, H, tw, th, Gd, E, scal);
-      launch_check("ncc_maps");
-      M = reinterpret_cast<const char *>(Gd);
-      mstep = W * 4;
-   } else {
-      map_sum<<<nscal, kThreads, 0, s>>>(static_cast<const char *>(G.ptr), G.step, W, H, scal);
-      launch_check("map_sum");
-      M = static_cast<const char *>(G.ptr);
-      mstep = G.step;
-   }
-
-   if (dT) {
-      // a multiple of 4 floats = 4 mod 32 (16-byte window loads, rows on different banks),
-      // at least tile + tw + 3
-      int ldsw = (tile + tw - 1 + 4 + 3) / 4 * 4;
-      while (ldsw % 32 != 4) ldsw += 4;
-      const size_t lds = (size_t)tile * tile * 4 + (size_t)(tile + th - 1) * ldsw * 4;
-      dim3 grid(nbx, nby);
-      if (tile == 64) {
-         hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(weight_corr<64>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), "attr");
-         weight_corr<64><<<grid, kThreads, lds, s>>>(M, mstep, Sp, S.step, W, H, tw, th, partial, ldsw);
-      } else {
-         if (lds > 160 * 1024) fail(CL_INVALID_VALUE, "template too large for the gradient");
-         hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(weight_corr<32>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), "attr");
-         weight_corr<32><<<grid, kThreads, lds, s>>>(M, mstep, Sp, S.step, W, H, tw, th, partial, ldsw);
-      }
-      launch_check("weight_corr");
-      grad_template<<<(tw * th + 63) / 64, kThreads, 0, s>>>(
-          partial, nbx * nby, scal, nscal, tstage, tpitch, tsums, tw, th, mode, dT, dtstep);
-      launch_check("grad_template");
-   }
-
-   if (dS) {
+   bool any_ds = false;
+   for (int i = 0; i < n; ++i) any_ds = any_ds || dS[i];
+   if (any_ds) {
       flip_template<<<grid_for((long long)tpitch * th), kThreads, 0, s>>>(tstage, tpitch, tw, th, tflip);
       launch_check("flip_template");
-      View mv{M, W, H, mstep, 1, SImage::F32};
-      launch_crosscorr_f32_anchored(s, mv, tflip, tpitch, tw, th, tw - 1 - tw / 2, th - 1 - th / 2,
-                                    F, W * 4);
-      // integral of the map whose flipped window sums multiply s
-      View bv = mode == 0 ? View{E, W, H, W * 4, 1, SImage::F32} : G;
-      launch_integral(s, bv, 0, false, I, 0, -1, aux);
-      const float alpha = mode == 0 ? 1.f : -2.f, beta = mode == 0 ? -1.f : 2.f;
-      grad_source<<<grid_for((long long)px), kThreads, 0, s>>>(F, Sp, S.step, I, W, H, tw, th,
-                                                               alpha, beta, dS, dsstep);
-      launch_check("grad_source");
+   }
+   for (int i = 0; i < n; ++i) {
+      const float *Sp = static_cast<const float *>(S[i].ptr);
+      if (G1) {
+         // forward CC with the same order as ocipCrossCorr, and the windows' sums of s^2
+         launch_crosscorr_f32_anchored(s, S[i], tstage, tpitch, tw, th, tw / 2, th / 2, F, W * 4);
+         if (use_box)
+            launch_box_f64(s, static_cast<const char *>(S[i].ptr), S[i].step, W, H, tw, th,
+                           tw / 2, th / 2, true, I);
+         else
+            launch_integral(s, S[i], 0, true, I, 0, -1, aux);
+      }
+      grad_maps<<<nscal, kThreads, 0, s>>>(
+          F, W * 4, G1 ? static_cast<const char *>(G1[i].ptr) : nullptr, G1 ? G1[i].step : 0,
+          G2 ? static_cast<const char *>(G2[i].ptr) : nullptr, G2 ? G2[i].step : 0,
+          use_box ? I : nullptr, use_box ? nullptr : I, tsums, W, H, tw, th, M, E,
+          sncc + (size_t)i * nscal, sssd + (size_t)i * nscal);
+      launch_check("grad_maps");
+      const char *Mc = reinterpret_cast<const char *>(M);
+
+      if (dT) {
+         float *pi = partial + (size_t)i * L.nb * taps;
+         if (tgrad_on_mfma(tw, th)) {
+            // the contraction on MFMA (split BF16); the FP64 reduction follows the batch
+            const int tasks_x = (W + kTgCols - 1) / kTgCols;
+            tgrad_mfma<<<L.nb, kThreads, 0, s>>>(Mc, W * 4, Sp, S[i].step, W, H, tw, th, tasks_x, pi);
+            launch_check("tgrad_mfma");
+         } else {
+            const int tile = wc_tile(tw, th);
+            // a multiple of 4 floats = 4 mod 32 (16-byte window loads, rows on different
+            // banks), at least tile + tw + 3
+            int ldsw = (tile + tw - 1 + 4 + 3) / 4 * 4;
+            while (ldsw % 32 != 4) ldsw += 4;
+            const size_t lds = (size_t)tile * tile * 4 + (size_t)(tile + th - 1) * ldsw * 4;
+            dim3 grid((W + tile - 1) / tile, (H + tile - 1) / tile);
+            if (tile == 64) {
+               hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(weight_corr<64>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), "attr");
+               weight_corr<64><<<grid, kThreads, lds, s>>>(Mc, W * 4, Sp, S[i].step, W, H, tw, th, pi, ldsw);
+            } else {
+               if (lds > 160 * 1024) fail(CL_INVALID_VALUE, "template too large for the gradient");
+               hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(weight_corr<32>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), "attr");
+               weight_corr<32><<<grid, kThreads, lds, s>>>(Mc, W * 4, Sp, S[i].step, W, H, tw, th, pi, ldsw);
+            }
+            launch_check("weight_corr");
+         }
+      }
+
+      if (dS[i]) {
+         View mv{M, W, H, W * 4, 1, SImage::F32};
+         launch_crosscorr_f32_anchored(s, mv, tflip, tpitch, tw, th, tw - 1 - tw / 2,
+                                       th - 1 - th / 2, F, W * 4);
+         // flipped window sums of Eb (window anchored at tw - 1 - cx, th - 1 - cy)
+         View ev{E, W, H, W * 4, 1, SImage::F32};
+         if (use_box)
+            launch_box_f64(s, reinterpret_cast<const char *>(E), W * 4, W, H, tw, th,
+                           tw - 1 - tw / 2, th - 1 - th / 2, false, I);
+         else
+            launch_integral(s, ev, 0, false, I, 0, -1, aux);
+         grad_source<<<grid_for((long long)px), kThreads, 0, s>>>(
+             F, Sp, S[i].step, use_box ? I : nullptr, use_box ? nullptr : I, W, H, tw, th, dS[i],
+             dsstep[i]);
+         launch_check("grad_source");
+      }
+   }
+   if (dT) {
+      grad_template<<<(taps + 63) / 64, kThreads, 0, s>>>(
+          partial, n * L.nb, sncc, G1 ? n * nscal : 0, sssd, G2 ? n * nscal : 0, tstage, tpitch,
+          tsums, tw, th, dT, dtstep);
+      launch_check("grad_template");
    }
 }
 

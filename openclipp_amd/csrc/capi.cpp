@@ -195,51 +195,84 @@ void run_proximity(int op, ocipImage hs, ocipImage ht, ocipImage hd, int precisi
    D.in_device = true;
 }
 
-// Gradients of L = sum(G * op(S, T)) (extensions; F32 1-channel images).
-void run_backward(int mode, ocipImage hs, ocipImage ht, ocipImage hg, ocipImage hds,
-                  ocipImage hdt)
+// Gradients (extensions; F32 1-channel images) of
+//   L = sum_i [ sum(G1_i * CrossCorr_Norm(S_i, T)) + sum(G2_i * SqrDistance(S_i, T)) ]
+// for a batch of same-size sources sharing one template: dS_i and dT (summed over the batch);
+// g1 / g2 null: that op absent; ds entries and dt may be null.
+void run_backward_batch(uint n, const ocipImage *hs, ocipImage ht, const ocipImage *g1,
+                        const ocipImage *g2, const ocipImage *hds, ocipImage hdt)
 {
    Context &c = current();
-   Image &S = img(hs), &T = img(ht), &G = img(hg);
-   Image *dS = hds ? &img(hds) : nullptr;
-   Image *dT = hdt ? &img(hdt) : nullptr;
+   if (n == 0) return;
+   if (!hs || (!g1 && !g2)) fail(CL_INVALID_VALUE, "null source or gradient array");
    auto f32_1 = [](const Image &i) {
       return i.desc.Type == SImage::F32 && i.desc.Channels == 1;
    };
-   if (T.desc.Width > S.desc.Width || T.desc.Height > S.desc.Height)
-      fail(CL_INVALID_VALUE, "template larger than source");
-   if (!f32_1(S) || !f32_1(T) || !f32_1(G)) fail(CL_INVALID_VALUE, "F32 1-channel images");
-   check_same_size(S, G);
-   if (dS) {
-      if (!f32_1(*dS)) fail(CL_INVALID_VALUE, "F32 1-channel images");
-      check_same_size(S, *dS);
-   }
+   Image &T = img(ht);
+   if (!f32_1(T)) fail(CL_INVALID_VALUE, "F32 1-channel images");
+   if (T.desc.Width == 0 || T.desc.Height == 0) fail(CL_INVALID_VALUE, "empty template");
+   same_context(c, T);
+   Image *dT = hdt ? &img(hdt) : nullptr;
    if (dT) {
       if (!f32_1(*dT)) fail(CL_INVALID_VALUE, "F32 1-channel images");
       check_same_size(T, *dT);
+      same_context(c, *dT);
    }
-   if (T.desc.Width == 0 || T.desc.Height == 0) fail(CL_INVALID_VALUE, "empty template");
-   same_context(c, S); same_context(c, T); same_context(c, G);
-   if (dS) same_context(c, *dS);
-   if (dT) same_context(c, *dT);
+   std::vector<Image *> S(n), G1(n), G2(n), dS(n);
+   for (uint i = 0; i < n; ++i) {
+      S[i] = &img(hs[i]);
+      if (!f32_1(*S[i])) fail(CL_INVALID_VALUE, "F32 1-channel images");
+      if (T.desc.Width > S[i]->desc.Width || T.desc.Height > S[i]->desc.Height)
+         fail(CL_INVALID_VALUE, "template larger than source");
+      if (i > 0) check_same_size(*S[0], *S[i]);
+      same_context(c, *S[i]);
+      G1[i] = g1 ? &img(g1[i]) : nullptr;
+      G2[i] = g2 ? &img(g2[i]) : nullptr;
+      dS[i] = hds && hds[i] ? &img(hds[i]) : nullptr;
+      for (Image *m : {G1[i], G2[i], dS[i]}) {
+         if (!m) continue;
+         if (!f32_1(*m)) fail(CL_INVALID_VALUE, "F32 1-channel images");
+         check_same_size(*S[i], *m);
+         same_context(c, *m);
+      }
+   }
    set_device(c);
-   send_if_needed(c, S);
    send_if_needed(c, T);
-   send_if_needed(c, G);
-   const int tw = T.desc.Width, th = T.desc.Height, W = S.desc.Width, H = S.desc.Height;
+   std::vector<View> vs(n), v1(n), v2(n);
+   std::vector<char *> ds(n);
+   std::vector<int> dss(n);
+   for (uint i = 0; i < n; ++i) {
+      send_if_needed(c, *S[i]);
+      vs[i] = view(*S[i]);
+      if (G1[i]) { send_if_needed(c, *G1[i]); v1[i] = view(*G1[i]); }
+      if (G2[i]) { send_if_needed(c, *G2[i]); v2[i] = view(*G2[i]); }
+      ds[i] = dS[i] ? static_cast<char *>(dS[i]->dev) : nullptr;
+      dss[i] = dS[i] ? (int)dS[i]->desc.Step : 0;
+   }
+   const int tw = T.desc.Width, th = T.desc.Height;
+   const int W = S[0]->desc.Width, H = S[0]->desc.Height;
    const int tpitch = tpitch_for(tw);
    c.tmpl.data.ensure((size_t)tpitch * th * sizeof(float));
    c.tmpl.sums.ensure(2 * sizeof(double));
-   const double *integ = mode == 0 ? sqr_integral_for(c, S, 0) : nullptr;
    c.integ_aux.ensure(integral_aux_bytes(W, H));
    launch_template_prep(c.stream, view(T), 0, tpitch, 0, c.tmpl.data.ptr, c.tmpl.sums.as<double>());
-   c.work.ensure(backward_work_bytes(W, H, tw, th, tpitch));
-   launch_backward(c.stream, mode, view(S), view(G), c.tmpl.data.as<float>(), tpitch, tw, th,
-                   c.tmpl.sums.as<double>(), integ, c.work.ptr, c.integ_aux.ptr,
-                   dS ? static_cast<char *>(dS->dev) : nullptr, dS ? (int)dS->desc.Step : 0,
-                   dT ? static_cast<char *>(dT->dev) : nullptr, dT ? (int)dT->desc.Step : 0);
-   if (dS) dS->in_device = true;
+   c.work.ensure(backward_work_bytes((int)n, W, H, tw, th, tpitch));
+   launch_backward_batch(c.stream, (int)n, vs.data(), g1 ? v1.data() : nullptr,
+                         g2 ? v2.data() : nullptr, c.tmpl.data.as<float>(), tpitch, tw, th,
+                         c.tmpl.sums.as<double>(), c.work.ptr, c.integ_aux.ptr, ds.data(),
+                         dss.data(), dT ? static_cast<char *>(dT->dev) : nullptr,
+                         dT ? (int)dT->desc.Step : 0);
+   for (uint i = 0; i < n; ++i)
+      if (dS[i]) dS[i]->in_device = true;
    if (dT) dT->in_device = true;
+}
+
+// One image, one op (mode 0: CrossCorr_Norm, 1: SqrDistance).
+void run_backward(int mode, ocipImage hs, ocipImage ht, ocipImage hg, ocipImage hds,
+                  ocipImage hdt)
+{
+   run_backward_batch(1, &hs, ht, mode == 0 ? &hg : nullptr, mode == 1 ? &hg : nullptr, &hds,
+                      hdt);
 }
 
 // Many templates against one source (extensions).  F32 precision runs the plain path per
@@ -656,6 +689,16 @@ ocipError ocip_API ocipSqrDistanceBackward(ocipImage Source, ocipImage Template,
                                            ocipImage GradTemplate)
 {
    return guarded([&] { run_backward(1, Source, Template, GradDest, GradSource, GradTemplate); });
+}
+
+ocipError ocip_API ocipProximityBackwardBatch(const ocipImage *Sources, ocipImage Template,
+                                              const ocipImage *GradNcc, const ocipImage *GradSsd,
+                                              ocipImage *GradSources, ocipImage GradTemplate,
+                                              uint Count)
+{
+   return guarded([&] {
+      run_backward_batch(Count, Sources, Template, GradNcc, GradSsd, GradSources, GradTemplate);
+   });
 }
 
 // ---- statistics -------------------------------------------------------------------------

@@ -223,11 +223,14 @@ void run_batch_mfma(hipStream_t s, BatchCache *cache, int num_cus, int precision
                     const View &S, const std::vector<View> &T, const std::vector<float *> &dst,
                     const std::vector<int> &dstep);
 
-// Gradients (backward.hip).  mode 0: CrossCorr_Norm, 1: SqrDistance.  dS / dT may be null.
-size_t backward_work_bytes(int W, int H, int tw, int th, int tpitch);
-void launch_backward(hipStream_t s, int mode, const View &S, const View &G, const float *tstage,
-                     int tpitch, int tw, int th, const double *tsums, const double *integ_ss,
-                     void *work, void *aux, char *dS, int dsstep, char *dT, int dtstep);
+// Gradients (backward.hip) of L = sum_i [sum G1_i CrossCorr_Norm(S_i, t) + sum G2_i
+// SqrDistance(S_i, t)] over n same-size F32 1-channel images: G1 / G2 null leaves that op out;
+// dS[i] / dT may be null.  dT is the sum over the batch.
+size_t backward_work_bytes(int n, int W, int H, int tw, int th, int tpitch);
+void launch_backward_batch(hipStream_t s, int n, const View *S, const View *G1, const View *G2,
+                           const float *tstage, int tpitch, int tw, int th, const double *tsums,
+                           void *work, void *aux, char *const *dS, const int *dsstep, char *dT,
+                           int dtstep);
 
 // Proximity through 2-D FFTs (FP64): op kSSD, kSSDNorm, kCC or kCCNorm on an F32 1-channel
 // source and template into an F32 dest.  integ: the exclusive FP64 sum-of-squares integral

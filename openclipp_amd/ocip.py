@@ -99,6 +99,7 @@ SIGNATURES = {
     "ocipSqrDistance_NormBatch": [_vp, _ptr_vp, _ptr_vp, ctypes.c_uint, ctypes.c_int],
     "ocipCrossCorr_NormBackward": [_vp, _vp, _vp, _vp, _vp],
     "ocipSqrDistanceBackward": [_vp, _vp, _vp, _vp, _vp],
+    "ocipProximityBackwardBatch": [_ptr_vp, _vp, _ptr_vp, _ptr_vp, _ptr_vp, _vp, ctypes.c_uint],
     "ocipEnableKernelTiming": [ctypes.c_char],
     "ocipGetKernelTime": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                           ctypes.POINTER(ctypes.c_uint)],

@@ -9,7 +9,8 @@ here, in the test harness only.  Checks:
   FP64 forward (scipy FFT correlation) at 64 seeded coordinates (32 source pixels, 32
   template taps), with L = sum(w1 * NCC) + sum(w2 * SSD), w ~ N(0, 1); and all 16 images'
   full gradients against an FP64 reference gradient (torch float64 autograd through an FFT
-  forward).
+  forward), one call per image and op, and in one ocipProximityBackwardBatch call;
+* ocipProximityBackwardBatch against the single-image, single-op calls added.
 """
 import numpy as np
 import pytest
@@ -259,3 +260,89 @@ def test_diff_1024_all_images_against_fp64_reference(ocip):
     err_t = np.max(np.abs(dT_got - dT_want)) / np.abs(dT_want).max()
     say("diff_1024 relative error: dS", worst_s, "dT", float(err_t))
     assert err_t <= 2e-4
+
+
+def _batch_backward(ocip, srcs, tpl, g_ncc, g_ssd, want_ds=True, want_dt=True):
+    n = len(srcs)
+    Ss = [ocip.Image(s) for s in srcs]
+    T = ocip.Image(tpl)
+    G1 = [ocip.Image(g) for g in g_ncc] if g_ncc is not None else None
+    G2 = [ocip.Image(g) for g in g_ssd] if g_ssd is not None else None
+    dS = [ocip.Image(shape=srcs[0].shape, dtype=np.float32) for _ in range(n)] if want_ds else None
+    dT = ocip.Image(shape=tpl.shape, dtype=np.float32) if want_dt else None
+    ocip.check(ocip.lib.ocipProximityBackwardBatch(
+        ocip.handles(Ss), T.handle, ocip.handles(G1) if G1 else None,
+        ocip.handles(G2) if G2 else None, ocip.handles(dS) if dS else None,
+        dT.handle if dT else None, n), "ocipProximityBackwardBatch")
+    ds = [d.read().astype(np.float64) for d in dS] if dS else None
+    dt = dT.read().astype(np.float64) if dT else None
+    return ds, dt
+
+
+def _single_backward(ocip, src, tpl, g, fn):
+    S, T, G = ocip.Image(src), ocip.Image(tpl), ocip.Image(g)
+    dS = ocip.Image(shape=src.shape, dtype=np.float32)
+    dT = ocip.Image(shape=tpl.shape, dtype=np.float32)
+    ocip.check(fn(S.handle, T.handle, G.handle, dS.handle, dT.handle))
+    return dS.read().astype(np.float64), dT.read().astype(np.float64)
+
+
+@pytest.mark.parametrize("tw,th", [(9, 7), (31, 31), (40, 36)])
+def test_batch_backward_equals_single_calls(ocip, tw, th):
+    """ocipProximityBackwardBatch over 3 images, both ops: dS_i and the batch-summed dT equal
+    the single-image, single-op calls added, within the rounding of one combined map against
+    two: 2e-6 of the largest component for dS (FP32 correlations), 1e-5 for dT on the FP32
+    contraction (40 x 36: FP32 partial sums of 4096 products per tile), 5e-5 for dT on the
+    split-BF16 contraction (up to 32 x 32), whose products carry up to 2^-14 relative error;
+    with one op absent (GradNcc or GradSsd NULL) the same."""
+    rng = np.random.default_rng(tw * 100 + th)
+    n, H, W = 3, 130, 170
+    srcs = [rng.random((H, W)).astype(np.float32) for _ in range(n)]
+    tpl = rng.normal(0, 1, (th, tw)).astype(np.float32)
+    g1 = [rng.normal(0, 1, (H, W)).astype(np.float32) for _ in range(n)]
+    g2 = [rng.normal(0, 1, (H, W)).astype(np.float32) for _ in range(n)]
+    singles = [(_single_backward(ocip, srcs[i], tpl, g1[i], ocip.lib.ocipCrossCorr_NormBackward),
+                _single_backward(ocip, srcs[i], tpl, g2[i], ocip.lib.ocipSqrDistanceBackward))
+               for i in range(n)]
+    for which in ("both", "ncc", "ssd"):
+        say("batch backward", tw, th, which)
+        ds, dt = _batch_backward(ocip, srcs, tpl, g1 if which != "ssd" else None,
+                                 g2 if which != "ncc" else None)
+        want_t = sum((a[1] if which != "ssd" else 0) + (b[1] if which != "ncc" else 0)
+                     for a, b in singles)
+        tol_t = 5e-5 if tw <= 32 and th <= 32 else 1e-5
+        assert np.max(np.abs(dt - want_t)) <= tol_t * np.abs(want_t).max(), which
+        for i in range(n):
+            a, b = singles[i]
+            want_s = (a[0] if which != "ssd" else 0) + (b[0] if which != "ncc" else 0)
+            assert np.max(np.abs(ds[i] - want_s)) <= 2e-6 * np.abs(want_s).max(), (which, i)
+
+
+def test_batch_backward_diff_1024_against_fp64_reference(ocip):
+    """The whole 16 x 1024x1024 / 31x31 workload in one ocipProximityBackwardBatch call (both
+    ops): every dS_i and the summed dT within 2e-4 of the largest component of the FP64
+    reference gradient of L = sum w1 NCC + sum w2 SSD."""
+    from openclipp_amd import workloads
+    srcs, tpl, w_ncc, w_ssd = workloads.diff_1024x16()
+    say("batch backward diff_1024")
+    ds, dt = _batch_backward(ocip, list(srcs), tpl, list(w_ncc), list(w_ssd))
+    want_t = np.zeros(tpl.shape)
+    for k in range(len(srcs)):
+        want_s, wt = ref_grad_fft(srcs[k], tpl, w_ncc[k], w_ssd[k])
+        want_t += wt
+        err = np.max(np.abs(ds[k] - want_s)) / np.abs(want_s).max()
+        assert err <= 2e-4, (k, err)
+    assert np.max(np.abs(dt - want_t)) / np.abs(want_t).max() <= 2e-4
+
+
+def test_batch_backward_errors(ocip):
+    src = np.zeros((40, 50), np.float32)
+    S, T = ocip.Image(src), ocip.Image(np.ones((5, 5), np.float32))
+    G = ocip.Image(src)
+    Gbad = ocip.Image(np.zeros((40, 51), np.float32))
+    L = ocip.lib.ocipProximityBackwardBatch
+    assert L(ocip.handles([S]), T.handle, None, None, None, None, 1) == ocip.CL_INVALID_VALUE
+    assert L(ocip.handles([S]), T.handle, ocip.handles([Gbad]), None, None, None, 1) == \
+        ocip.CL_INVALID_IMAGE_SIZE
+    assert L(None, T.handle, None, None, None, None, 0) == ocip.CL_SUCCESS
+    assert L(ocip.handles([S]), T.handle, ocip.handles([G]), None, None, None, 1) == ocip.CL_SUCCESS
