@@ -446,12 +446,17 @@ __global__ __launch_bounds__(kThreads) void grad_template(const float *partial, 
    }
 }
 
-// flipped staged template: tf(u, v) = t(tw-1-u, th-1-v)
-__global__ void flip_template(const float *tstage, int tpitch, int tw, int th, float *tflip)
+// Flipped staged template with a leading zero column when tw is even and a leading zero row
+// when th is even (twf = tw + ox, thf = th + oy): tf(u, v) = t(tw-1-(u-ox), th-1-(v-oy)).
+// Correlating a map with tf at anchor (twf/2, thf/2) is correlating it with the flipped
+// template at anchor (tw-1-tw/2, th-1-th/2).
+__global__ void flip_template(const float *tstage, int tpitch, int tw, int th, int twf, int thf,
+                              float *tflip)
 {
-   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < tpitch * th; i += gridDim.x * blockDim.x) {
-      const int v = i / tpitch, u = i - v * tpitch;
-      tflip[i] = u < tw ? tstage[(size_t)(th - 1 - v) * tpitch + (tw - 1 - u)] : 0.f;
+   const int ox = twf - tw, oy = thf - th;
+   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < twf * thf; i += gridDim.x * blockDim.x) {
+      const int v = i / twf - oy, u = i % twf - ox;
+      tflip[i] = (u >= 0 && v >= 0) ? tstage[(size_t)(th - 1 - v) * tpitch + (tw - 1 - u)] : 0.f;
    }
 }
 
@@ -476,9 +481,15 @@ __global__ __launch_bounds__(kThreads) void grad_source(const float *F, const fl
 
 int grid_for(long long n) { return (int)std::min<long long>(1024, (n + kThreads - 1) / kThreads); }
 
+// Images of one chunk: their CrossCorr maps (then their dS correlations), M and Eb maps are
+// held together so that each correlation runs on the whole chunk in one launch.
+constexpr int kChunkPixels = 16 << 20;
+
 struct BwLayout {
-   size_t F, M, E, I, partial, scal, tflip, total;
-   int nb;   // template-gradient partial blocks per image
+   size_t F, M, E, I, partial, scal, tflip, fragT, fragF, sums, total;
+   int nb;        // template-gradient partial blocks per image
+   int zc;        // images per chunk
+   int twf, thf;  // flipped template, zero-padded to odd sizes
 };
 
 size_t align_up(size_t n) { return (n + 255) / 256 * 256; }
@@ -497,19 +508,27 @@ int tgrad_blocks(int W, int H, int tw, int th)
    return ((W + t - 1) / t) * ((H + t - 1) / t);
 }
 
-BwLayout bw_layout(int n, int W, int H, int tw, int th, int tpitch)
+BwLayout bw_layout(int n, int W, int H, int tw, int th)
 {
    const size_t px = (size_t)W * H;
    BwLayout L;
    L.nb = tgrad_blocks(W, H, tw, th);
+   L.zc = (int)std::max<size_t>(1, std::min<size_t>({(size_t)std::max(n, 1), (size_t)kSplitImages,
+                                                     kChunkPixels / std::max<size_t>(px, 1)}));
+   L.twf = tw + (tw % 2 == 0);
+   L.thf = th + (th % 2 == 0);
+   const size_t maps = (size_t)L.zc * px * sizeof(float);
    size_t o = 0;
-   L.F = o;       o = align_up(o + px * sizeof(float));
-   L.M = o;       o = align_up(o + px * sizeof(float));
-   L.E = o;       o = align_up(o + px * sizeof(float));
+   L.F = o;       o = align_up(o + maps);
+   L.M = o;       o = align_up(o + maps);
+   L.E = o;       o = align_up(o + maps);
    L.I = o;       o = align_up(o + excl_bytes(W, H));    // a dense W x H FP64 map fits too
    L.partial = o; o = align_up(o + (size_t)n * L.nb * tw * th * sizeof(float));
    L.scal = o;    o = align_up(o + (size_t)2 * n * 1024 * sizeof(double));
-   L.tflip = o;   o = align_up(o + (size_t)tpitch * th * sizeof(float));
+   L.tflip = o;   o = align_up(o + (size_t)L.twf * L.thf * sizeof(float));
+   L.fragT = o;   o = align_up(o + crosscorr_split_frag_bytes(tw, th));
+   L.fragF = o;   o = align_up(o + crosscorr_split_frag_bytes(L.twf, L.thf));
+   L.sums = o;    o = align_up(o + 8 * sizeof(double));
    L.total = o;
    return L;
 }
@@ -518,11 +537,15 @@ BwLayout bw_layout(int n, int W, int H, int tw, int th, int tpitch)
 
 size_t backward_work_bytes(int n, int W, int H, int tw, int th, int tpitch)
 {
-   return bw_layout(n, W, H, tw, th, tpitch).total;
+   (void)tpitch;
+   return bw_layout(n, W, H, tw, th).total;
 }
 
 // Gradients of L = sum_i [sum G1_i NCC(S_i, t) + sum G2_i SSD(S_i, t)] (G1 / G2 null: that op
-// absent): dS_i for the non-null dS[i], dT (if not null) summed over the batch.
+// absent): dS_i for the non-null dS[i], dT (if not null) summed over the batch.  Images go in
+// chunks: the forward CrossCorr maps of a chunk (split-BF16 MFMA), then per image the window
+// sums of s^2, the maps M and Eb and the template-gradient contraction, then the chunk's
+// correlations of M with the flipped template (split-BF16 MFMA), then per image dS.
 void launch_backward_batch(hipStream_t s, int n, const View *S, const View *G1, const View *G2,
                            const float *tstage, int tpitch, int tw, int th, const double *tsums,
                            void *work, void *aux, char *const *dS, const int *dsstep, char *dT,
@@ -530,16 +553,17 @@ void launch_backward_batch(hipStream_t s, int n, const View *S, const View *G1, 
 {
    const int W = S[0].width, H = S[0].height;
    const size_t px = (size_t)W * H;
-   const BwLayout L = bw_layout(n, W, H, tw, th, tpitch);
+   const BwLayout L = bw_layout(n, W, H, tw, th);
    char *w = static_cast<char *>(work);
-   float *F = reinterpret_cast<float *>(w + L.F);
-   float *M = reinterpret_cast<float *>(w + L.M);
-   float *E = reinterpret_cast<float *>(w + L.E);
+   float *F0 = reinterpret_cast<float *>(w + L.F);
+   float *M0 = reinterpret_cast<float *>(w + L.M);
+   float *E0 = reinterpret_cast<float *>(w + L.E);
    double *I = reinterpret_cast<double *>(w + L.I);
    float *partial = reinterpret_cast<float *>(w + L.partial);
    double *sncc = reinterpret_cast<double *>(w + L.scal);
    double *sssd = sncc + (size_t)n * 1024;
    float *tflip = reinterpret_cast<float *>(w + L.tflip);
+   double *sums = reinterpret_cast<double *>(w + L.sums);
    const int nscal = grid_for((long long)px);
    const int taps = tw * th;
    // window sums straight from the maps (box_f64) when its LDS tile fits, else from integrals
@@ -548,30 +572,51 @@ void launch_backward_batch(hipStream_t s, int n, const View *S, const View *G1, 
 
    bool any_ds = false;
    for (int i = 0; i < n; ++i) any_ds = any_ds || dS[i];
-   if (any_ds) {
-      flip_template<<<grid_for((long long)tpitch * th), kThreads, 0, s>>>(tstage, tpitch, tw, th, tflip);
-      launch_check("flip_template");
+   if (G1) {
+      const View tv{tstage, tw, th, tpitch * (int)sizeof(float), 1, SImage::F32};
+      prep_crosscorr_split(s, tv, w + L.fragT, sums);
    }
-   for (int i = 0; i < n; ++i) {
-      const float *Sp = static_cast<const float *>(S[i].ptr);
+   if (any_ds) {
+      flip_template<<<grid_for((long long)L.twf * L.thf), kThreads, 0, s>>>(tstage, tpitch, tw, th,
+                                                                           L.twf, L.thf, tflip);
+      launch_check("flip_template");
+      const View fv{tflip, L.twf, L.thf, L.twf * (int)sizeof(float), 1, SImage::F32};
+      prep_crosscorr_split(s, fv, w + L.fragF, sums + 4);
+   }
+   const float *src[kSplitImages];
+   float *dst[kSplitImages];
+   int step[kSplitImages];
+   for (int c0 = 0; c0 < n; c0 += L.zc) {
+      const int m = std::min(L.zc, n - c0);
       if (G1) {
-         // forward CC with the same order as ocipCrossCorr, and the windows' sums of s^2
-         launch_crosscorr_f32_anchored(s, S[i], tstage, tpitch, tw, th, tw / 2, th / 2, F, W * 4);
-         if (use_box)
-            launch_box_f64(s, static_cast<const char *>(S[i].ptr), S[i].step, W, H, tw, th,
-                           tw / 2, th / 2, true, I);
-         else
-            launch_integral(s, S[i], 0, true, I, 0, -1, aux);
+         // the chunk's forward CrossCorr maps
+         for (int k = 0; k < m; ++k) {
+            src[k] = static_cast<const float *>(S[c0 + k].ptr);
+            step[k] = S[c0 + k].step;
+            dst[k] = F0 + (size_t)k * px;
+         }
+         launch_crosscorr_split(s, m, src, step, dst, W * 4, W, H, tw, th, w + L.fragT, sums);
       }
-      grad_maps<<<nscal, kThreads, 0, s>>>(
-          F, W * 4, G1 ? static_cast<const char *>(G1[i].ptr) : nullptr, G1 ? G1[i].step : 0,
-          G2 ? static_cast<const char *>(G2[i].ptr) : nullptr, G2 ? G2[i].step : 0,
-          use_box ? I : nullptr, use_box ? nullptr : I, tsums, W, H, tw, th, M, E,
-          sncc + (size_t)i * nscal, sssd + (size_t)i * nscal);
-      launch_check("grad_maps");
-      const char *Mc = reinterpret_cast<const char *>(M);
-
-      if (dT) {
+      for (int k = 0; k < m; ++k) {
+         const int i = c0 + k;
+         const float *Sp = static_cast<const float *>(S[i].ptr);
+         float *F = F0 + (size_t)k * px, *M = M0 + (size_t)k * px, *E = E0 + (size_t)k * px;
+         if (G1) {
+            // the windows' sums of s^2
+            if (use_box)
+               launch_box_f64(s, static_cast<const char *>(S[i].ptr), S[i].step, W, H, tw, th,
+                              tw / 2, th / 2, true, I);
+            else
+               launch_integral(s, S[i], 0, true, I, 0, -1, aux);
+         }
+         grad_maps<<<nscal, kThreads, 0, s>>>(
+             F, W * 4, G1 ? static_cast<const char *>(G1[i].ptr) : nullptr, G1 ? G1[i].step : 0,
+             G2 ? static_cast<const char *>(G2[i].ptr) : nullptr, G2 ? G2[i].step : 0,
+             use_box ? I : nullptr, use_box ? nullptr : I, tsums, W, H, tw, th, M, E,
+             sncc + (size_t)i * nscal, sssd + (size_t)i * nscal);
+         launch_check("grad_maps");
+         if (!dT) continue;
+         const char *Mc = reinterpret_cast<const char *>(M);
          float *pi = partial + (size_t)i * L.nb * taps;
          if (tgrad_on_mfma(tw, th)) {
             // the contraction on MFMA (split BF16); the FP64 reduction follows the batch
@@ -599,18 +644,30 @@ void launch_backward_batch(hipStream_t s, int n, const View *S, const View *G1, 
             launch_check("weight_corr");
          }
       }
-
-      if (dS[i]) {
-         View mv{M, W, H, W * 4, 1, SImage::F32};
-         launch_crosscorr_f32_anchored(s, mv, tflip, tpitch, tw, th, tw - 1 - tw / 2,
-                                       th - 1 - th / 2, F, W * 4);
+      // the chunk's correlations of M with the flipped template, into F
+      int md = 0;
+      for (int k = 0; k < m; ++k) {
+         if (!dS[c0 + k]) continue;
+         src[md] = M0 + (size_t)k * px;
+         step[md] = W * 4;
+         dst[md] = F0 + (size_t)k * px;
+         ++md;
+      }
+      if (md) launch_crosscorr_split(s, md, src, step, dst, W * 4, W, H, L.twf, L.thf, w + L.fragF,
+                                     sums + 4);
+      for (int k = 0; k < m; ++k) {
+         const int i = c0 + k;
+         if (!dS[i]) continue;
+         const float *Sp = static_cast<const float *>(S[i].ptr);
+         float *F = F0 + (size_t)k * px, *E = E0 + (size_t)k * px;
          // flipped window sums of Eb (window anchored at tw - 1 - cx, th - 1 - cy)
-         View ev{E, W, H, W * 4, 1, SImage::F32};
-         if (use_box)
+         if (use_box) {
             launch_box_f64(s, reinterpret_cast<const char *>(E), W * 4, W, H, tw, th,
                            tw - 1 - tw / 2, th - 1 - th / 2, false, I);
-         else
+         } else {
+            const View ev{E, W, H, W * 4, 1, SImage::F32};
             launch_integral(s, ev, 0, false, I, 0, -1, aux);
+         }
          grad_source<<<grid_for((long long)px), kThreads, 0, s>>>(
              F, Sp, S[i].step, use_box ? I : nullptr, use_box ? nullptr : I, W, H, tw, th, dS[i],
              dsstep[i]);

@@ -793,6 +793,63 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
    if (norm) launch_ncc_scale(s, S, tw, th, tsums, integ, dst, dstep);
 }
 
+// ======================================================================= many sources
+// Split-BF16 CrossCorr maps (anchor (tw/2, th/2), zero outside the image, no Norm) of several
+// sources of one size with one template: crosscorr_mfma with 2 column tiles, the
+// images of up to kZImages sources in one launch.  The template's fragments are prepared once
+// by prep_crosscorr_split.
+static_assert(kSplitImages == mfma::kZImages, "sources per launch");
+size_t crosscorr_split_frag_bytes(int tw, int th) { return mfma_frag_bytes(mfma::kBF16x3, tw, th); }
+
+static Geometry split_geometry(int tw, int th)
+{
+   const Geometry g = geometry(mfma::kBF16x3, kc_for(tw), 2, th);
+   if (g.vb < 1) fail(CL_INVALID_VALUE, "template too large for the MFMA tile");
+   return g;
+}
+
+void prep_crosscorr_split(hipStream_t s, const View &T, void *frags, double *sums)
+{
+   const int tw = T.width, th = T.height, KC = kc_for(tw), nub = nub_for(tw);
+   (void)split_geometry(tw, th);
+   u16x8 *f0 = static_cast<u16x8 *>(frags);
+   const BatchDesc d{static_cast<const char *>(T.ptr), tw, th, T.step, KC, nub, f0,
+                     f0 + (size_t)nub * th * KC * 64, sums};
+   mfma::launch_single_prep<mfma::kBF16x3>(s, d);
+}
+
+void launch_crosscorr_split(hipStream_t s, int n, const float *const *src, const int *sstep,
+                            float *const *dst, int dstep, int W, int H, int tw, int th,
+                            const void *frags, const double *sums)
+{
+   const int KC = kc_for(tw), nub = nub_for(tw);
+   const Geometry g = split_geometry(tw, th);
+   const u16x8 *f0 = static_cast<const u16x8 *>(frags);
+   const u16x8 *f1 = f0 + (size_t)nub * th * KC * 64;
+   KernelTimer kt(s, "crosscorr_mfma");
+   for (int i0 = 0; i0 < n; i0 += mfma::kZImages) {
+      SingleLaunch L;
+      L.z.n = std::min(mfma::kZImages, n - i0);
+      // 16-byte staging loads: tw/2 a multiple of 4 and every source's rows 16-byte aligned
+      int vec = (tw / 2) % 4 == 0;
+      for (int k = 0; k < L.z.n; ++k) {
+         L.z.src[k] = src[i0 + k];
+         L.z.dst[k] = dst[i0 + k];
+         L.z.sstep[k] = sstep[i0 + k];
+         if (reinterpret_cast<uintptr_t>(src[i0 + k]) % 16 || sstep[i0 + k] % 16) vec = 0;
+      }
+      L.mode = mfma::kModeCC;
+      L.a = mfma::TileArgs{src[i0], W, H, sstep[i0], tw, th, nub, f0, f1, sums, nullptr, dst[i0],
+                           dstep, g.vb, vec};
+      L.KC = KC;
+      L.NT = 2;
+      L.ldsp = g.ldsp;
+      L.lds = g.lds;
+      L.grid = dim3((W + 63) / 64, (H + mfma::kTileRows - 1) / mfma::kTileRows);
+      mfma::launch_single<mfma::kBF16x3>(s, L);
+   }
+}
+
 // ======================================================================= exact integer sums
 // 8-bit integer images through the row-pair kernel (kModeInt): the exact cross term of
 // CrossCorr, CrossCorr_Norm, SqrDistance and SqrDistance_Norm on BF16 MFMA, and the window

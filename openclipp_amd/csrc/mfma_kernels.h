@@ -65,6 +65,17 @@ struct TileArgs {
    int src8 = 0, op = 0, flush = 0;
 };
 
+// Several sources of one size and template with crosscorr_mfma in one launch: the source, its
+// row pitch and the output of image z < n (n = 0: the one source of TileArgs).  Passed by
+// value among the kernel arguments.
+constexpr int kZImages = 16;
+struct ZImages {
+   const float *src[kZImages];
+   float *dst[kZImages];
+   int sstep[kZImages];
+   int n;
+};
+
 // One template of a batch (device table).
 struct BatchItem {
    int tw, th, nub, vb;
@@ -427,14 +438,21 @@ __device__ __forceinline__ void mfma_tile(const TileArgs &a, int bx, int by, uns
 
 // LDS use limits these kernels to two workgroups per CU (two waves per SIMD), so the compiler
 // may spend registers on keeping two template rows of B fragments in flight.
-// One-dimensional grid of tiles_x * tiles_y workgroups in XCD-aware order.
+// One-dimensional grid of tiles_x * tiles_y workgroups per image in XCD-aware order (images
+// z.n of them, one after the other, or the one image of `a`).
 template <int P, int KC, int NT, int MODE>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(NT == 2 ? 3 : 1, NT == 2 ? 3 : 2)))
-void crosscorr_mfma(TileArgs a, int ldsp, int tiles_x, int tiles)
+void crosscorr_mfma(TileArgs a, int ldsp, int tiles_x, int tiles, ZImages z)
 {
    extern __shared__ u16x8 lds_v[];
-   const int t = xcd_tile(blockIdx.x, tiles);
+   const int zi = blockIdx.x / tiles;
+   const int t = xcd_tile(blockIdx.x - zi * tiles, tiles);
+   if (z.n) {
+      a.src = z.src[zi];
+      a.dst = z.dst[zi];
+      a.sstep = z.sstep[zi];
+   }
    mfma_tile<P, KC, NT, MODE>(a, t % tiles_x, t / tiles_x, reinterpret_cast<unsigned short *>(lds_v),
                               ldsp);
 }
@@ -1078,6 +1096,7 @@ struct SingleLaunch {
    int KC, NT, mode, ldsp;
    size_t lds;
    dim3 grid;
+   ZImages z{};   // crosscorr_mfma over several sources (z.n > 0; grid per image)
 };
 
 struct BatchLaunch {

@@ -181,11 +181,6 @@ void launch_proximity(hipStream_t s, int op, const View &src, const View &tpl, v
                       int dst_step, const void *tstage, int tpitch, const double *tsums,
                       const double *integ, int channel);
 
-// FP32 cross-correlation of a 1-channel F32 map with a staged float template whose origin
-// sits at (-ax, -ay) (the ops above use ax = tw/2, ay = th/2).
-void launch_crosscorr_f32_anchored(hipStream_t s, const View &M, const float *tstage, int tpitch,
-                                   int tw, int th, int ax, int ay, float *dst, int dstep);
-
 // MFMA cross-correlation (F32 1-channel source and template), precision 1..3.
 // The template's fragments and sum of squares (tsums[0]) are built from T in the call.
 // may_tune = false: no autotuning (no timing events); the variant tuned earlier for this
@@ -195,6 +190,15 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
                            int dst_step, void *work, bool may_tune = true);
 // The single-template MFMA Norm path forms its denominators from the source image; only
 // templates wider than 513 columns read the FP64 sum-of-squares integral instead.
+// Split-BF16 CrossCorr maps (no Norm) of n sources of one size (row pitches sstep[]) with one
+// template, whose fragments (crosscorr_split_frag_bytes) prep_crosscorr_split wrote to
+// `frags` (and sum t^2 to sums[0]); up to 16 sources per launch
+constexpr int kSplitImages = 16;   // sources per launch of launch_crosscorr_split
+size_t crosscorr_split_frag_bytes(int tw, int th);
+void prep_crosscorr_split(hipStream_t s, const View &T, void *frags, double *sums);
+void launch_crosscorr_split(hipStream_t s, int n, const float *const *src, const int *sstep,
+                            float *const *dst, int dstep, int W, int H, int tw, int th,
+                            const void *frags, const double *sums);
 bool crosscorr_mfma_needs_integral(int tw);
 size_t crosscorr_mfma_work_bytes(int precision, int width, int height, int tw, int th);
 

@@ -551,39 +551,6 @@ static void launch_op(hipStream_t s, const View &S, const View &T, void *dst, in
 #undef OCIP_TILE_INT
 }
 
-template <int R>
-static void crosscorr_f32_anchored(hipStream_t s, const View &M, const float *tstage, int tpitch,
-                                   int tw, int th, int ax, int ay, float *dst, int dstep)
-{
-   const int W = M.width, H = M.height;
-   const int ncols = kWaves * R + tw - 1;
-   const int ldsw = lds_pitch(ncols);
-   const int rows_fit = kLdsBudget / (ldsw * 4) - (kTileRows - 1);
-   if (rows_fit < 1 || M.type != SImage::F32 || M.channels != 1)
-      fail(CL_INVALID_VALUE, "anchored correlation: unsupported shape");
-   const int vb = min(th, rows_fit);
-   const size_t lds = (size_t)(kTileRows + vb - 1) * ldsw * 4;
-   dim3 grid((W + kWaves * R - 1) / (kWaves * R), (H + kTileRows - 1) / kTileRows);
-   KernelTimer kt(s, "prox_tile_f32");
-   prox_tile_f32<kCC, SImage::F32, R><<<grid.x * grid.y, 256, lds, s>>>(
-       static_cast<const char *>(M.ptr), W, H, M.step, 1, 0, tstage, tpitch, tw, th, nullptr,
-       nullptr, reinterpret_cast<char *>(dst), dstep, vb, ldsw, ax, ay);
-   launch_check("prox_tile_f32");
-}
-
-void launch_crosscorr_f32_anchored(hipStream_t s, const View &M, const float *tstage, int tpitch,
-                                   int tw, int th, int ax, int ay, float *dst, int dstep)
-{
-   // 16 outputs per lane, unless that leaves fewer than two workgroups per CU (images below
-   // about 2048 x 1024): then 8, with twice the workgroups
-   const long long tiles16 = (long long)((M.width + kWaves * 16 - 1) / (kWaves * 16)) *
-                             ((M.height + kTileRows - 1) / kTileRows);
-   if (tiles16 >= 512)
-      crosscorr_f32_anchored<16>(s, M, tstage, tpitch, tw, th, ax, ay, dst, dstep);
-   else
-      crosscorr_f32_anchored<8>(s, M, tstage, tpitch, tw, th, ax, ay, dst, dstep);
-}
-
 void launch_proximity(hipStream_t s, int op, const View &S, const View &T, void *dst, int dstep,
                       const void *tstage, int tpitch, const double *tsums, const double *integ,
                       int ch)
