@@ -372,13 +372,13 @@ Geometry geometry(int precision, int KC, int NT, int th_max)
 // (16 NT + 4) FP32 tile of the epilogue, whichever is larger.
 int kc2_for(int tw) { return tw <= 97 ? (tw + 15 + 15) / 16 : (mfma::kBlockW + 15 + 15) / 16; }
 
-Geometry pair_geometry(int KC, int NT, int th)
+Geometry pair_geometry(int KC, int NT, int th, long lds_cap = mfma::kLdsMax)
 {
    Geometry g;
    const int lcols = mfma::pair_lcols(KC, NT);
    g.ldsp = (lcols + 7) / 8 * 8;
    while ((g.ldsp * 2) % 128 != 16) g.ldsp += 8;
-   const long rows_fit = (long)mfma::kLdsMax / ((long)g.ldsp * 2);
+   const long rows_fit = lds_cap / ((long)g.ldsp * 2);
    const int S = (th + 1) / 2;
    g.vb = (int)std::min<long>(S, (rows_fit - (mfma::kTileRows - 2)) / 2);
    const size_t staging = (size_t)(mfma::kTileRows - 2 + 2 * g.vb) * g.ldsp * 2;
@@ -472,6 +472,18 @@ void launch_batch_p(int precision, hipStream_t s, const BatchLaunch &L)
    if (precision == mfma::kBF16) mfma::launch_batch<mfma::kBF16>(s, L);
    else if (precision == mfma::kF16) mfma::launch_batch<mfma::kF16>(s, L);
    else mfma::launch_batch<mfma::kBF16x3>(s, L);
+}
+
+void launch_pair_batch_p(int precision, hipStream_t s, const BatchLaunch &L)
+{
+   if (precision == mfma::kBF16) mfma::launch_pair_batch<mfma::kBF16>(s, L);
+   else mfma::launch_pair_batch<mfma::kF16>(s, L);
+}
+
+void launch_pair_batch_prep_p(int precision, hipStream_t s, const BatchDesc *d, int n)
+{
+   if (precision == mfma::kBF16) mfma::launch_pair_batch_prep<mfma::kBF16>(s, d, n);
+   else mfma::launch_pair_batch_prep<mfma::kF16>(s, d, n);
 }
 
 void launch_batch_prep_p(int precision, hipStream_t s, const BatchDesc *d, int n)
@@ -1080,20 +1092,31 @@ void run_batch_mfma(hipStream_t s, BatchCache *cache, int num_cus, int precision
       return;
    }
 
+   // BF16 and FP16 run the row-pair tile (crosscorr_mfma_pair_batch; KC = kc2_for(tw) chunks,
+   // a step = two template rows), split BF16 the tile of crosscorr_mfma (KC = kc_for(tw))
+   const bool pair = precision != mfma::kBF16x3;
+   auto kc_of = [&](int tw) { return pair ? kc2_for(tw) : kc_for(tw); };
+   auto nfrag_of = [&](const View &t) {
+      const int nub = nub_for(t.width);
+      return (size_t)nub * (pair ? mfma::pair_frag_steps(t.height, nub) : t.height) * kc_of(t.width);
+   };
+   // the row-pair batch's LDS per workgroup: at most half a CU's, two workgroups per CU
+   const long pair_cap = mfma::kLdsMax / 2;
+
    // ---- build: buckets by KC, templates in each bucket by decreasing cost
    std::map<int, std::vector<int>> buckets;
-   for (int i = 0; i < n; ++i) buckets[kc_for(T[i].width)].push_back(i);
+   for (int i = 0; i < n; ++i) buckets[kc_of(T[i].width)].push_back(i);
    for (auto &b : buckets)
       std::stable_sort(b.second.begin(), b.second.end(), [&](int a, int c) {
          return (long)T[a].height * nub_for(T[a].width) > (long)T[c].height * nub_for(T[c].width);
       });
    if (buckets.size() > 64) fail(CL_INVALID_VALUE, "too many template widths");
-   const int planes = planes_for(precision);
+   const int planes = pair ? 1 : planes_for(precision);
    std::vector<size_t> frag_off(n);
    size_t nfrag_total = 0;
    for (int i = 0; i < n; ++i) {
       frag_off[i] = nfrag_total;
-      nfrag_total += (size_t)nub_for(T[i].width) * T[i].height * kc_for(T[i].width);
+      nfrag_total += nfrag_of(T[i]);
    }
    BatchEntry *e = new BatchEntry();
    try {
@@ -1111,7 +1134,7 @@ void run_batch_mfma(hipStream_t s, BatchCache *cache, int num_cus, int precision
       std::vector<BatchDesc> descs(n);
       for (int i = 0; i < n; ++i)
          descs[i] = BatchDesc{static_cast<const char *>(T[i].ptr), T[i].width, T[i].height,
-                              T[i].step, kc_for(T[i].width), nub_for(T[i].width),
+                              T[i].step, kc_of(T[i].width), nub_for(T[i].width),
                               f0 + frag_off[i] * 64, f1 ? f1 + frag_off[i] * 64 : nullptr,
                               sums + 2 * i};
       hip_check(hipMemcpy(e->descs.ptr, descs.data(), n * sizeof(BatchDesc),
@@ -1126,12 +1149,17 @@ void run_batch_mfma(hipStream_t s, BatchCache *cache, int num_cus, int precision
       for (auto &b : buckets) {
          Bucket k{b.first, (int)items.size(), (int)b.second.size(), 0};
          for (int i : b.second) k.th_max = std::max(k.th_max, T[i].height);
-         const Geometry g = geometry(precision, k.KC, 4, k.th_max);
+         // rows (pair: steps) per LDS band for the widest tile a bucket may run with (NT = 4,
+         // pair: 6), so that every tile width the autotuner tries fits
+         const Geometry g = pair ? pair_geometry(k.KC, 6, k.th_max, pair_cap)
+                                 : geometry(precision, k.KC, 4, k.th_max);
          if (g.vb < 1) fail(CL_INVALID_VALUE, "template too large for the MFMA tile");
-         for (int i : b.second)
+         for (int i : b.second) {
+            const int depth = pair ? (T[i].height + 1) / 2 : T[i].height;
             items.push_back(BatchItem{T[i].width, T[i].height, nub_for(T[i].width),
-                                      std::min(T[i].height, g.vb), descs[i].frag0,
+                                      std::min(depth, g.vb), descs[i].frag0,
                                       descs[i].frag1, sums + 2 * i, dst[i], dstep[i]});
+         }
          bl.push_back(k);
       }
       hip_check(hipMemcpy(e->items.ptr, items.data(), n * sizeof(BatchItem),
@@ -1141,7 +1169,8 @@ void run_batch_mfma(hipStream_t s, BatchCache *cache, int num_cus, int precision
       const BatchItem *ditems = e->items.as<BatchItem>();
       auto bucket_launch = [&](size_t b, int NT) {
          const Bucket &k = bl[b];
-         const Geometry g = geometry(precision, k.KC, NT, k.th_max);
+         const Geometry g = pair ? pair_geometry(k.KC, NT, k.th_max, pair_cap)
+                                 : geometry(precision, k.KC, NT, k.th_max);
          BatchLaunch L;
          L.src = static_cast<const float *>(S.ptr);
          L.W = S.width;
@@ -1151,8 +1180,9 @@ void run_batch_mfma(hipStream_t s, BatchCache *cache, int num_cus, int precision
          L.integ = e->integ.as<double>();
          L.items = ditems + k.first;
          L.nitems = k.count;
-         L.tiles_x = (S.width + 32 * NT - 1) / (32 * NT);
-         L.tiles_y = (S.height + mfma::kTileRows - 1) / mfma::kTileRows;
+         const int tcols = pair ? 16 * NT : 32 * NT, trows = pair ? mfma::kPairRows : mfma::kTileRows;
+         L.tiles_x = (S.width + tcols - 1) / tcols;
+         L.tiles_y = (S.height + trows - 1) / trows;
          L.counter = counters + b;
          L.KC = k.KC;
          L.NT = NT;
@@ -1161,11 +1191,13 @@ void run_batch_mfma(hipStream_t s, BatchCache *cache, int num_cus, int precision
          L.lds = g.lds;
          const int per_cu = std::max(1, std::min(2, (int)(mfma::kLdsMax / std::max<size_t>(g.lds, 1))));
          L.grid = num_cus * per_cu;
-         launch_batch_p(precision, s, L);
+         if (pair) launch_pair_batch_p(precision, s, L);
+         else launch_batch_p(precision, s, L);
       };
       auto prep = [&]() {
          launch_integral(s, S, 0, true, e->integ.ptr, 0, -1, e->aux.ptr);
-         launch_batch_prep_p(precision, s, e->descs.as<BatchDesc>(), n);
+         if (pair) launch_pair_batch_prep_p(precision, s, e->descs.as<BatchDesc>(), n);
+         else launch_batch_prep_p(precision, s, e->descs.as<BatchDesc>(), n);
       };
       auto reset = [&]() {
          reset_counters<<<1, 64, 0, s>>>(counters, (int)bl.size());
@@ -1184,7 +1216,10 @@ void run_batch_mfma(hipStream_t s, BatchCache *cache, int num_cus, int precision
             };
             char shape[48];
             snprintf(shape, sizeof shape, "kc%d:thmax%d", bl[b].KC, bl[b].th_max);
-            nts.push_back(tuned_variant(tune_key("batch", precision, mode, shape, S), s, run, {4, 2}));
+            if (pair)
+               nts.push_back(tuned_variant(tune_key("pair_batch", precision, mode, shape, S), s, run, {6, 4}));
+            else
+               nts.push_back(tuned_variant(tune_key("batch", precision, mode, shape, S), s, run, {4, 2}));
          }
       }
       // record the whole batch once

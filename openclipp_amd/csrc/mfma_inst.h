@@ -60,6 +60,24 @@ template <int KC> void strip_mode(hipStream_t s, const SingleLaunch &L, int tile
    else strip<KC, false>(s, L, tiles_y, q_rows);
 }
 
+template <int KC, int NT, int MODE> void pair_batch(hipStream_t s, const BatchLaunch &L)
+{
+   auto k = crosscorr_mfma_pair_batch<P, KC, NT, MODE>;
+   allow_lds(k);
+   k<<<L.grid, 256, L.lds, s>>>(L.src, L.W, L.H, L.sstep, L.vec, L.integ, L.items, L.nitems,
+                                L.tiles_x, L.tiles_y, L.counter, L.ldsp);
+}
+
+template <int KC, int MODE> void pair_batch_nt(hipStream_t s, const BatchLaunch &L)
+{
+   if (L.NT == 4) pair_batch<KC, 4, MODE>(s, L); else pair_batch<KC, 6, MODE>(s, L);
+}
+
+template <int KC> void pair_batch_mode(hipStream_t s, const BatchLaunch &L)
+{
+   if (L.mode == kModeNCC) pair_batch_nt<KC, kModeNCC>(s, L); else pair_batch_nt<KC, kModeSSDN>(s, L);
+}
+
 template <int KC> void pair_mode(hipStream_t s, const SingleLaunch &L)
 {
    if (L.mode == kModeInt) {
@@ -139,6 +157,27 @@ template <> void launch_pair<P>(hipStream_t s, const SingleLaunch &L)
    launch_check("crosscorr_mfma_pair");
 }
 
+template <> void launch_pair_batch<P>(hipStream_t s, const BatchLaunch &L)
+{
+   switch (L.KC) {
+   case 1: pair_batch_mode<1>(s, L); break;
+   case 2: pair_batch_mode<2>(s, L); break;
+   case 3: pair_batch_mode<3>(s, L); break;
+   case 4: pair_batch_mode<4>(s, L); break;
+   case 5: pair_batch_mode<5>(s, L); break;
+   case 6: pair_batch_mode<6>(s, L); break;
+   case 7: pair_batch_mode<7>(s, L); break;
+   default: fail(CL_INVALID_VALUE, "template width");
+   }
+   launch_check("crosscorr_mfma_pair_batch");
+}
+
+template <> void launch_pair_batch_prep<P>(hipStream_t s, const BatchDesc *descs, int n)
+{
+   batch_prep<P, true><<<dim3(kPrepBlocks, n), 256, 0, s>>>(descs);
+   launch_check("batch_prep");
+}
+
 // grid.x = workgroups, grid.y = tiles_x
 template <> void launch_strip<P>(hipStream_t s, const SingleLaunch &L, int tiles_y, int q_rows)
 {
@@ -170,7 +209,7 @@ template <> void launch_single_prep<P>(hipStream_t s, const BatchDesc &d)
 
 template <> void launch_batch_prep<P>(hipStream_t s, const BatchDesc *descs, int n)
 {
-   batch_prep<P><<<n, 256, 0, s>>>(descs);
+   batch_prep<P, false><<<dim3(kPrepBlocks, n), 256, 0, s>>>(descs);
    launch_check("batch_prep");
 }
 

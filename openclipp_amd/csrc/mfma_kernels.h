@@ -189,19 +189,6 @@ __device__ __forceinline__ void template_sqsum(const BatchDesc &d, double *red)
    if (threadIdx.x == 0) d.sums[0] = red[0] + red[1] + red[2] + red[3];
 }
 
-// Batch: one workgroup per template builds its fragments and its FP64 sum of squares.
-template <int P>
-__global__ __launch_bounds__(256) void batch_prep(const BatchDesc *__restrict__ descs)
-{
-   __shared__ double red[4];
-   const BatchDesc d = descs[blockIdx.x];
-   const int nfrag = d.nub * d.th * d.KC;
-   for (int k = threadIdx.x; k < nfrag * 64; k += 256)
-      build_fragment<P>(d.tptr, d.tstep, d.tw, d.th, d.KC, d.nub, k >> 6, k & 63, d.frag0,
-                        d.frag1);
-   template_sqsum(d, red);
-}
-
 // Single template straight from its image: one fragment per wavefront, the FP64 sum of
 // squares (same order as batch_prep) in the last block, alongside.
 template <int P>
@@ -558,6 +545,29 @@ __global__ __launch_bounds__(256) void pair_prep(BatchDesc d)
                              d.frag0, d.src8);
 }
 
+// Batch: for template blockIdx.y, workgroups 0 .. gridDim.x - 2 build its fragments (in the
+// row-pair layout when PAIR), the last one its FP64 sum of squares.
+constexpr int kPrepBlocks = 17;
+template <int P, bool PAIR>
+__global__ __launch_bounds__(256) void batch_prep(const BatchDesc *__restrict__ descs)
+{
+   __shared__ double red[4];
+   const BatchDesc d = descs[blockIdx.y];
+   if (blockIdx.x == gridDim.x - 1) {
+      template_sqsum(d, red);
+      return;
+   }
+   const int nfrag = d.nub * (PAIR ? pair_frag_steps(d.th, d.nub) : d.th) * d.KC;
+   for (int k = blockIdx.x * 256 + threadIdx.x; k < nfrag * 64; k += (gridDim.x - 1) * 256) {
+      if constexpr (PAIR)
+         build_pair_fragment<P>(d.tptr, d.tstep, d.tw, d.th, d.KC, d.nub, k >> 6, k & 63,
+                                d.frag0, d.src8);
+      else
+         build_fragment<P>(d.tptr, d.tstep, d.tw, d.th, d.KC, d.nub, k >> 6, k & 63, d.frag0,
+                           d.frag1);
+   }
+}
+
 // One step: NT + KC - 1 A fragments from the LDS row `ab`, KC B fragments.
 template <int P, int KC, int NT>
 __device__ __forceinline__ void pair_mfma(f32x16 (&acc)[NT], const unsigned short *ab,
@@ -573,8 +583,11 @@ __device__ __forceinline__ void pair_mfma(f32x16 (&acc)[NT], const unsigned shor
       for (int j = 0; j < NT; ++j) acc[j] = mma<P>(a0[j + kc], b[kc], acc[j]);
 }
 
-// Output tile: 127 rows x 16 NT columns (the CrossCorr map); a.vb = steps (template row pairs)
-// per LDS band.
+// Output tile (bx, by): 127 rows x 16 NT columns; a.vb = steps (template row pairs) per LDS
+// band.  Starts with a barrier, so consecutive calls may reuse the LDS.  kModeCC stores the
+// CrossCorr map; kModeNCC / kModeSSDN store CrossCorr_Norm / SqrDistance_Norm with the window
+// sums of s^2 from the FP64 integral a.integ (the batch kernel's epilogue, the same
+// expressions as mfma_tile).
 //
 // kModeInt (8-bit integer images, BF16 operands): every value and product is an integer that
 // BF16 and FP32 hold exactly, and so is every partial sum below 2^24.  Each step adds one
@@ -585,22 +598,17 @@ __device__ __forceinline__ void pair_mfma(f32x16 (&acc)[NT], const unsigned shor
 // (and the window sum of squares, exact in int32, which ss_box8 wrote into the map before)
 // with the same FP64 expressions as prox_tile_int, so the maps equal that kernel's bit for bit.
 template <int P, int KC, int NT, int MODE>
-__global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(NT <= 4 && MODE == kModeCC ? 3 : 2,
-                                   NT <= 4 && MODE == kModeCC ? 3 : 2)))
-void crosscorr_mfma_pair(TileArgs a, int ldsp, int tiles_x, int tiles)
+__device__ __forceinline__ void pair_tile(const TileArgs &a, int bx, int by, unsigned short *lds,
+                                          int ldsp)
 {
    static_assert(P != kBF16x3, "one operand plane");
-   static_assert(MODE == kModeCC || (MODE == kModeInt && P == kBF16 && NT == 4),
-                 "the CrossCorr map, or exact integer sums (BF16, 4 n-tiles)");
+   static_assert(MODE == kModeCC || MODE == kModeNCC || MODE == kModeSSDN ||
+                     (MODE == kModeInt && P == kBF16 && NT == 4),
+                 "float maps, or exact integer sums (BF16, 4 n-tiles)");
    constexpr bool INT = MODE == kModeInt;
    constexpr int lcols = pair_lcols(KC, NT);
    constexpr int C = 16 * NT;
    constexpr int TP = pair_tile_pitch(NT);
-   extern __shared__ u16x8 lds_v[];
-   unsigned short *lds = reinterpret_cast<unsigned short *>(lds_v);
-   const int t = xcd_tile(blockIdx.x, tiles);
-   const int bx = t % tiles_x, by = t / tiles_x;
    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
    const int x0 = bx * C, Y0 = by * kPairRows;
    const int cx = a.tw / 2, cy = a.th / 2;
@@ -755,13 +763,31 @@ void crosscorr_mfma_pair(TileArgs a, int ldsp, int tiles_x, int tiles)
    // thread (the tile pitch keeps them 16-byte aligned in LDS too), one at a time at the
    // right edge; otherwise consecutive threads take consecutive columns
    static_assert(TP % 4 == 0 && C % 4 == 0, "16-byte groups of the LDS tile");
+   double tt = 0.0;
+   if constexpr (MODE != kModeCC) tt = a.tsums[0];
+   // the stored value of output (x, y) from its CrossCorr sum c
+   auto out = [&](float c, int x, int y) -> float {
+      if constexpr (MODE == kModeCC) {
+         return c;
+      } else {
+         const double ss = window_sqsum(a.integ, a.W, a.H, x, y, a.tw, a.th);
+         const double num = MODE == kModeNCC ? (double)c : ss - 2.0 * (double)c + tt;
+         return (float)num * inv_sqrt(ss * tt);
+      }
+   };
    if (a.dstep % 16 == 0 && reinterpret_cast<uintptr_t>(a.dst) % 16 == 0) {
       for (int k = threadIdx.x; k < kPairRows * (C / 4); k += 256) {
          const int r = k / (C / 4), c = 4 * (k - r * (C / 4));
          const int y = Y0 + r, x = x0 + c;
          if (y >= a.H || x >= a.W) continue;
          float *o = reinterpret_cast<float *>(reinterpret_cast<char *>(a.dst) + (size_t)y * a.dstep) + x;
-         const float4 q = *reinterpret_cast<const float4 *>(&tile[(r + 1) * TP + c]);
+         float4 q = *reinterpret_cast<const float4 *>(&tile[(r + 1) * TP + c]);
+         if constexpr (MODE != kModeCC) {
+            q.x = out(q.x, x, y);
+            q.y = out(q.y, x + 1, y);
+            q.z = out(q.z, x + 2, y);
+            q.w = out(q.w, x + 3, y);
+         }
          if (x + 4 <= a.W) {
             *reinterpret_cast<float4 *>(o) = q;
          } else {
@@ -776,8 +802,50 @@ void crosscorr_mfma_pair(TileArgs a, int ldsp, int tiles_x, int tiles)
          const int y = Y0 + r, x = x0 + c;
          if (y >= a.H || x >= a.W) continue;
          reinterpret_cast<float *>(reinterpret_cast<char *>(a.dst) + (size_t)y * a.dstep)[x] =
-             tile[(r + 1) * TP + c];
+             out(tile[(r + 1) * TP + c], x, y);
       }
+   }
+}
+
+// One 127 x 16 NT tile per workgroup, XCD-aware order (the single-template call).
+template <int P, int KC, int NT, int MODE>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(NT <= 4 && MODE == kModeCC ? 3 : 2,
+                                   NT <= 4 && MODE == kModeCC ? 3 : 2)))
+void crosscorr_mfma_pair(TileArgs a, int ldsp, int tiles_x, int tiles)
+{
+   extern __shared__ u16x8 lds_v[];
+   const int t = xcd_tile(blockIdx.x, tiles);
+   pair_tile<P, KC, NT, MODE>(a, t % tiles_x, t / tiles_x, reinterpret_cast<unsigned short *>(lds_v),
+                              ldsp);
+}
+
+// Persistent batch kernel on the row-pair tile (the batch of BF16 / FP16 templates): as
+// crosscorr_mfma_batch, items (template, tile) from `counter`, template-major with the
+// templates by decreasing cost; two workgroups per CU.
+template <int P, int KC, int NT, int MODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void crosscorr_mfma_pair_batch(const float *src, int W, int H, int sstep, int vec,
+                               const double *integ, const BatchItem *__restrict__ items,
+                               int nitems, int tiles_x, int tiles_y, unsigned *counter, int ldsp)
+{
+   extern __shared__ u16x8 lds_v[];
+   __shared__ int s_item;
+   const int tiles = tiles_x * tiles_y;
+   const int total = nitems * tiles;
+   for (;;) {
+      __syncthreads();
+      if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+      __syncthreads();
+      const int it = s_item;
+      if (it >= total) break;
+      const BatchItem b = items[it / tiles];
+      const int tile = it - (it / tiles) * tiles;
+      // 16-byte staging loads: aligned rows (vec) and tw/2 a multiple of 4
+      TileArgs a{src, W, H, sstep, b.tw, b.th, b.nub, b.frag0, nullptr, b.tsums, integ,
+                 b.dst, b.dstep, b.vb, (vec && (b.tw / 2) % 4 == 0) ? 1 : 0};
+      pair_tile<P, KC, NT, MODE>(a, tile % tiles_x, tile / tiles_x,
+                                 reinterpret_cast<unsigned short *>(lds_v), ldsp);
    }
 }
 
@@ -1118,6 +1186,9 @@ template <int P> void launch_pair(hipStream_t s, const SingleLaunch &L);
 template <int P> void launch_pair_prep(hipStream_t s, const BatchDesc &d);
 template <int P> void launch_strip(hipStream_t s, const SingleLaunch &L, int tiles_y, int q_rows);
 template <int P> void launch_batch_prep(hipStream_t s, const BatchDesc *descs, int n);
+// the batch on the row-pair tile (BF16, FP16): its kernel and its fragments
+template <int P> void launch_pair_batch(hipStream_t s, const BatchLaunch &L);
+template <int P> void launch_pair_batch_prep(hipStream_t s, const BatchDesc *descs, int n);
 
 }  // namespace mfma
 }  // namespace ocip

@@ -45,10 +45,13 @@ def run_batch(ocip, fn, src, tpls, prec, S=None, Ts=None, Ds=None):
 
 
 @pytest.fixture(autouse=True)
-def single_calls_use_the_batch_tile_code(monkeypatch):
-    """The single-template calls compared here run crosscorr_mfma (variant 4), the tile code
-    of the batch kernel, so that the per-pixel accumulation order is the same."""
-    monkeypatch.setenv("OCIP_MFMA_VARIANT", "4")
+def single_calls_use_the_batch_tile_code(monkeypatch, request):
+    """The single-template calls compared here run the tile code of the batch kernel at the
+    same precision, so that the per-pixel accumulation order is the same: the row-pair tile
+    (variant 16) for BF16 and FP16, crosscorr_mfma (variant 4) for split BF16."""
+    callspec = getattr(request.node, "callspec", None)
+    prec = callspec.params.get("prec") if callspec else None
+    monkeypatch.setenv("OCIP_MFMA_VARIANT", "4" if prec == "bf16x3" else "16")
 
 
 def same_map(batch, single, exact):
@@ -89,7 +92,7 @@ def test_ncc_batch_equals_single_calls(ocip, prec):
         assert np.all(np.abs(g - f64) <= bound * np.abs(f64) + 1e-6)
 
 
-@pytest.mark.parametrize("prec", ["bf16x3", "f32"])
+@pytest.mark.parametrize("prec", ["bf16", "f16", "bf16x3", "f32"])
 def test_ssd_norm_batch(ocip, prec):
     """SqrDistance_Norm: F32 runs the direct (s - t)^2 path; the MFMA precisions expand
     SS - 2 CC + TT, whose error is that of CC times 2 over the denominator."""
@@ -107,6 +110,34 @@ def test_ssd_norm_batch(ocip, prec):
         bound = (2 * (UNIT[prec] + 2 * (t.size + 16) * 2.0 ** -24) * sabs + 2.0 ** -40 * (ss + tt)) \
             / np.sqrt(ss * tt)
         assert np.all(np.abs(g - f64) <= bound * 1.01 + 1e-7)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "f16", "bf16x3"])
+def test_batch_wide_templates(ocip, prec):
+    """Templates wider than 97 columns (processed in 64-column blocks) mixed with narrow ones
+    in one batch, both ops: each map equals the single-template call to 1e-6 relative."""
+    rng = np.random.default_rng(21)
+    H, W = 150, 300
+    src = rng.random((H, W), dtype=np.float32)
+    tpls = [rng.random((th, tw), dtype=np.float32)
+            for tw, th in ((100, 9), (131, 20), (12, 33), (98, 5), (64, 64))]
+    for op, fn in (("CrossCorr_Norm", ocip.lib.ocipCrossCorr_NormBatch),
+                   ("SqrDistance_Norm", ocip.lib.ocipSqrDistance_NormBatch)):
+        say("wide batch", prec, op)
+        got, _ = run_batch(ocip, fn, src, tpls, prec)
+        for t, g in zip(tpls, got):
+            if op == "CrossCorr_Norm":
+                same_map(g, run_op(ocip, op, src, t, precision=PREC[prec]), False)
+            else:
+                # the single SqrDistance_Norm call has no MFMA form: against FP64
+                f64 = orc.prox_f64(orc.SSD_NORM, src, t)
+                s64, t64 = src.astype(np.float64), t.astype(np.float64)
+                ss = orc.prox_f64(orc.CC, s64 * s64, np.ones_like(t64))
+                tt = (t64 * t64).sum()
+                sabs = orc.prox_f64(orc.CC, np.abs(s64), np.abs(t64))
+                bound = (2 * (UNIT[prec] + 2 * (t.size + 16) * 2.0 ** -24) * sabs +
+                         2.0 ** -40 * (ss + tt)) / np.sqrt(ss * tt)
+                assert np.all(np.abs(g - f64) <= bound * 1.01 + 1e-7)
 
 
 def test_graph_replays_track_new_inputs(ocip):
