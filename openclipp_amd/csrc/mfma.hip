@@ -27,6 +27,7 @@ namespace {
 
 // work-queue counters of the batch kernels, cleared by a kernel (not a memset node) so that
 // the recorded graph replays correctly
+constexpr int kBatchCounters = 512;
 __global__ void reset_counters(unsigned *c, int n)
 {
    if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
@@ -1110,7 +1111,10 @@ void run_batch_mfma(hipStream_t s, BatchCache *cache, int num_cus, int precision
       std::stable_sort(b.second.begin(), b.second.end(), [&](int a, int c) {
          return (long)T[a].height * nub_for(T[a].width) > (long)T[c].height * nub_for(T[c].width);
       });
-   if (buckets.size() > 64) fail(CL_INVALID_VALUE, "too many template widths");
+   // one work-queue counter per bucket (and per XCD band on the row-pair tile)
+   const int counters_per_bucket = pair ? mfma::kPairBatchBands : 1;
+   if ((int)buckets.size() * counters_per_bucket > kBatchCounters)
+      fail(CL_INVALID_VALUE, "too many template widths");
    const int planes = pair ? 1 : planes_for(precision);
    std::vector<size_t> frag_off(n);
    size_t nfrag_total = 0;
@@ -1124,7 +1128,7 @@ void run_batch_mfma(hipStream_t s, BatchCache *cache, int num_cus, int precision
       e->sums.ensure((size_t)n * 2 * sizeof(double));
       e->descs.ensure((size_t)n * sizeof(BatchDesc));
       e->items.ensure((size_t)n * sizeof(BatchItem));
-      e->counters.ensure(64 * sizeof(unsigned));
+      e->counters.ensure(kBatchCounters * sizeof(unsigned));
       e->integ.ensure(excl_bytes(S.width, S.height));
       e->aux.ensure(integral_aux_bytes(S.width, S.height));
 
@@ -1183,7 +1187,7 @@ void run_batch_mfma(hipStream_t s, BatchCache *cache, int num_cus, int precision
          const int tcols = pair ? 16 * NT : 32 * NT, trows = pair ? mfma::kPairRows : mfma::kTileRows;
          L.tiles_x = (S.width + tcols - 1) / tcols;
          L.tiles_y = (S.height + trows - 1) / trows;
-         L.counter = counters + b;
+         L.counter = counters + b * counters_per_bucket;
          L.KC = k.KC;
          L.NT = NT;
          L.mode = mode;
@@ -1200,7 +1204,7 @@ void run_batch_mfma(hipStream_t s, BatchCache *cache, int num_cus, int precision
          else launch_batch_prep_p(precision, s, e->descs.as<BatchDesc>(), n);
       };
       auto reset = [&]() {
-         reset_counters<<<1, 64, 0, s>>>(counters, (int)bl.size());
+         reset_counters<<<1, kBatchCounters, 0, s>>>(counters, (int)bl.size() * counters_per_bucket);
          launch_check("reset_counters");
       };
 

@@ -216,14 +216,22 @@ __device__ __forceinline__ f32x16 mma(u16x8 a, u16x8 b, f32x16 c)
                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
+// The fragments are in device (global) memory.  Their pointers may come from a table in memory
+// (the batch kernels), where the compiler cannot infer the address space and would emit flat
+// loads; flat loads count against the LDS wait counter too, so waiting for the A fragments'
+// LDS reads would also wait for these prefetches.  Reading through global-address-space
+// pointers gives global loads, counted by vmcnt only.
+typedef const u16x8 __attribute__((address_space(1))) *gfrag_ptr;
+
 template <int NPL, int KC>
 __device__ __forceinline__ void load_frags(const u16x8 *__restrict__ f0, const u16x8 *__restrict__ f1,
                                            size_t base, u16x8 (&b0)[KC], u16x8 (&b1)[KC])
 {
+   const gfrag_ptr g0 = (gfrag_ptr)f0, g1 = (gfrag_ptr)f1;
 #pragma unroll
    for (int kc = 0; kc < KC; ++kc) {
-      b0[kc] = f0[base + (size_t)kc * 64];
-      if (NPL == 2) b1[kc] = f1[base + (size_t)kc * 64];
+      b0[kc] = g0[base + (size_t)kc * 64];
+      if (NPL == 2) b1[kc] = g1[base + (size_t)kc * 64];
    }
 }
 
@@ -462,7 +470,9 @@ void crosscorr_mfma_batch(const float *src, int W, int H, int sstep, int vec,
       __syncthreads();
       if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
       __syncthreads();
-      const int it = s_item;
+      // uniform across the workgroup: the item's table entry and every address derived from
+      // it stay in scalar registers
+      const int it = __builtin_amdgcn_readfirstlane(s_item);
       if (it >= total) break;
       const BatchItem b = items[it / tiles];
       const int tile = it - (it / tiles) * tiles;
@@ -683,6 +693,39 @@ __device__ __forceinline__ void pair_tile(const TileArgs &a, int bx, int by, uns
    }
    flush();
 
+   // Norm modes: groups of four consecutive outputs of one row per thread (kNormIters of them,
+   // kEpiGroups per batch).  A group's window sums of s^2 are differences of the FP64
+   // integral at rows ya, yb (shared by the four) and columns xa + e, xb + e, all clamped to
+   // the image, summed in the order of rect_sum.  The loads of the first batch go out before
+   // the halves are paired in LDS, and those of each next batch before a batch is used.
+   constexpr int kNormGroups = kPairRows * (C / 4);
+   constexpr int kNormIters = (kNormGroups + 255) / 256;
+   constexpr int kEpiGroups = 2;
+   constexpr bool NORM = MODE == kModeNCC || MODE == kModeSSDN;
+   typedef double Corners[kEpiGroups][4][4];   // [group][corner][e]
+   auto load_corners = [&](int i0, Corners &q) {
+      const size_t ip = excl_pitch(a.W);
+#pragma unroll
+      for (int u = 0; u < kEpiGroups; ++u) {
+         const int k = threadIdx.x + 256 * (i0 + u);
+         const int r = k / (C / 4), x = x0 + 4 * (k - r * (C / 4));
+         const int y = Y0 + r;
+         const int ya = min(max(y - cy, 0), a.H), yb = min(max(y - cy + a.th, 0), a.H);
+         const double *r0 = a.integ + (size_t)ya * ip + kExclOffset;
+         const double *r1 = a.integ + (size_t)yb * ip + kExclOffset;
+#pragma unroll
+         for (int e = 0; e < 4; ++e) {
+            const int xa = min(max(x + e - cx, 0), a.W), xb = min(max(x + e - cx + a.tw, 0), a.W);
+            q[u][0][e] = r0[xa];
+            q[u][1][e] = r1[xa];
+            q[u][2][e] = r0[xb];
+            q[u][3][e] = r1[xb];
+         }
+      }
+   };
+   Corners qa, qb;
+   if constexpr (NORM) load_corners(0, qa);
+
    // Pair the halves in LDS: tile row r = 32 w + m + dy holds output row Y0 - 1 + r.
    // C fragment: lane holds column n = lane % 32 of each n-tile, rows
    // m = 8 * (i / 4) + 4 * (lane / 32) + i % 4.  (kModeInt: int32 halves, exact sums.)
@@ -763,31 +806,55 @@ __device__ __forceinline__ void pair_tile(const TileArgs &a, int bx, int by, uns
    // thread (the tile pitch keeps them 16-byte aligned in LDS too), one at a time at the
    // right edge; otherwise consecutive threads take consecutive columns
    static_assert(TP % 4 == 0 && C % 4 == 0, "16-byte groups of the LDS tile");
-   double tt = 0.0;
-   if constexpr (MODE != kModeCC) tt = a.tsums[0];
-   // the stored value of output (x, y) from its CrossCorr sum c
-   auto out = [&](float c, int x, int y) -> float {
-      if constexpr (MODE == kModeCC) {
-         return c;
-      } else {
-         const double ss = window_sqsum(a.integ, a.W, a.H, x, y, a.tw, a.th);
-         const double num = MODE == kModeNCC ? (double)c : ss - 2.0 * (double)c + tt;
-         return (float)num * inv_sqrt(ss * tt);
+   if constexpr (NORM) {
+      const double tt = a.tsums[0];
+      const bool v4 = a.dstep % 16 == 0 && reinterpret_cast<uintptr_t>(a.dst) % 16 == 0;
+      auto finish = [&](int i0, const Corners &q) {
+#pragma unroll
+         for (int u = 0; u < kEpiGroups; ++u) {
+            const int k = threadIdx.x + 256 * (i0 + u);
+            const int r = k / (C / 4), c = 4 * (k - r * (C / 4));
+            const int y = Y0 + r, x = x0 + c;
+            if (k >= kNormGroups || y >= a.H || x >= a.W) continue;
+            const float4 t4 = *reinterpret_cast<const float4 *>(&tile[(r + 1) * TP + c]);
+            const float cc[4] = {t4.x, t4.y, t4.z, t4.w};
+            float v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+               const double ss = (q[u][3][e] - q[u][2][e]) - (q[u][1][e] - q[u][0][e]);
+               const double num = MODE == kModeNCC ? (double)cc[e] : ss - 2.0 * (double)cc[e] + tt;
+               v[e] = (float)num * inv_sqrt(ss * tt);
+            }
+            float *o = reinterpret_cast<float *>(reinterpret_cast<char *>(a.dst) + (size_t)y * a.dstep) + x;
+            if (v4 && x + 4 <= a.W) {
+               *reinterpret_cast<float4 *>(o) = make_float4(v[0], v[1], v[2], v[3]);
+            } else {
+#pragma unroll
+               for (int e = 0; e < 4; ++e)
+                  if (x + e < a.W) o[e] = v[e];
+            }
+         }
+      };
+      // two register sets alternate: the loads of batch i0 + kEpiGroups go out before batch i0
+      // is finished
+#pragma unroll
+      for (int i0 = 0; i0 < kNormIters; i0 += 2 * kEpiGroups) {
+         if (i0 + kEpiGroups < kNormIters) load_corners(i0 + kEpiGroups, qb);
+         finish(i0, qa);
+         if (i0 + kEpiGroups < kNormIters) {
+            if (i0 + 2 * kEpiGroups < kNormIters) load_corners(i0 + 2 * kEpiGroups, qa);
+            finish(i0 + kEpiGroups, qb);
+         }
       }
-   };
+      return;
+   }
    if (a.dstep % 16 == 0 && reinterpret_cast<uintptr_t>(a.dst) % 16 == 0) {
       for (int k = threadIdx.x; k < kPairRows * (C / 4); k += 256) {
          const int r = k / (C / 4), c = 4 * (k - r * (C / 4));
          const int y = Y0 + r, x = x0 + c;
          if (y >= a.H || x >= a.W) continue;
          float *o = reinterpret_cast<float *>(reinterpret_cast<char *>(a.dst) + (size_t)y * a.dstep) + x;
-         float4 q = *reinterpret_cast<const float4 *>(&tile[(r + 1) * TP + c]);
-         if constexpr (MODE != kModeCC) {
-            q.x = out(q.x, x, y);
-            q.y = out(q.y, x + 1, y);
-            q.z = out(q.z, x + 2, y);
-            q.w = out(q.w, x + 3, y);
-         }
+         const float4 q = *reinterpret_cast<const float4 *>(&tile[(r + 1) * TP + c]);
          if (x + 4 <= a.W) {
             *reinterpret_cast<float4 *>(o) = q;
          } else {
@@ -802,7 +869,7 @@ __device__ __forceinline__ void pair_tile(const TileArgs &a, int bx, int by, uns
          const int y = Y0 + r, x = x0 + c;
          if (y >= a.H || x >= a.W) continue;
          reinterpret_cast<float *>(reinterpret_cast<char *>(a.dst) + (size_t)y * a.dstep)[x] =
-             out(tile[(r + 1) * TP + c], x, y);
+             tile[(r + 1) * TP + c];
       }
    }
 }
@@ -820,9 +887,17 @@ void crosscorr_mfma_pair(TileArgs a, int ldsp, int tiles_x, int tiles)
                               ldsp);
 }
 
-// Persistent batch kernel on the row-pair tile (the batch of BF16 / FP16 templates): as
-// crosscorr_mfma_batch, items (template, tile) from `counter`, template-major with the
-// templates by decreasing cost; two workgroups per CU.
+// Persistent batch kernel on the row-pair tile (the batch of BF16 / FP16 templates).  The
+// tiles (row-major) are cut into kXcds * kSubBands bands of consecutive tiles, kSubBands per
+// XCD, each band with its own work-queue counter (counter[band]) over (template, tile of the
+// band) items, template-major with the templates by decreasing cost.  Workgroup b works
+// through the bands of XCD b % kXcds (the XCD that round-robin dispatch puts it on) in order,
+// so that an XCD's L2 holds the source rows, integral rows and template fragments of one
+// band's few current templates; then it takes items of the other XCDs' bands.  Two
+// workgroups per CU.
+constexpr int kSubBands = 4;
+constexpr int kPairBatchBands = kXcds * kSubBands;
+
 template <int P, int KC, int NT, int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void crosscorr_mfma_pair_batch(const float *src, int W, int H, int sstep, int vec,
@@ -832,20 +907,28 @@ void crosscorr_mfma_pair_batch(const float *src, int W, int H, int sstep, int ve
    extern __shared__ u16x8 lds_v[];
    __shared__ int s_item;
    const int tiles = tiles_x * tiles_y;
-   const int total = nitems * tiles;
-   for (;;) {
-      __syncthreads();
-      if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
-      __syncthreads();
-      const int it = s_item;
-      if (it >= total) break;
-      const BatchItem b = items[it / tiles];
-      const int tile = it - (it / tiles) * tiles;
-      // 16-byte staging loads: aligned rows (vec) and tw/2 a multiple of 4
-      TileArgs a{src, W, H, sstep, b.tw, b.th, b.nub, b.frag0, nullptr, b.tsums, integ,
-                 b.dst, b.dstep, b.vb, (vec && (b.tw / 2) % 4 == 0) ? 1 : 0};
-      pair_tile<P, KC, NT, MODE>(a, tile % tiles_x, tile / tiles_x,
-                                 reinterpret_cast<unsigned short *>(lds_v), ldsp);
+   const int home = (int)(blockIdx.x % kXcds) * kSubBands;
+   for (int rr = 0; rr < kPairBatchBands; ++rr) {
+      const int band = (home + rr) % kPairBatchBands;
+      const int t0 = (int)((long long)tiles * band / kPairBatchBands);
+      const int bt = (int)((long long)tiles * (band + 1) / kPairBatchBands) - t0;
+      const int total = nitems * bt;
+      for (;;) {
+         __syncthreads();
+         if (threadIdx.x == 0) s_item = total > 0 ? (int)atomicAdd(counter + band, 1u) : 0;
+         __syncthreads();
+         // uniform across the workgroup: the item's table entry and every address derived
+         // from it stay in scalar registers
+         const int it = __builtin_amdgcn_readfirstlane(s_item);
+         if (it >= total) break;
+         const BatchItem b = items[it / bt];
+         const int tile = t0 + (it - (it / bt) * bt);
+         // 16-byte staging loads: aligned rows (vec) and tw/2 a multiple of 4
+         TileArgs a{src, W, H, sstep, b.tw, b.th, b.nub, b.frag0, nullptr, b.tsums, integ,
+                    b.dst, b.dstep, b.vb, (vec && (b.tw / 2) % 4 == 0) ? 1 : 0};
+         pair_tile<P, KC, NT, MODE>(a, tile % tiles_x, tile / tiles_x,
+                                    reinterpret_cast<unsigned short *>(lds_v), ldsp);
+      }
    }
 }
 
