@@ -177,7 +177,9 @@ void run_proximity(int op, ocipImage hs, ocipImage ht, ocipImage hd, int precisi
    }
    for (int ch = 0; ch < nch; ++ch) {
       const bool mfma = precision != ocipPrecisionF32;
-      const bool need_integ = norm ? !mfma || crosscorr_mfma_needs_integral(tw) : ssd8 && !mfma;
+      const bool need_integ = norm ? (mfma ? crosscorr_mfma_needs_integral(tw)
+                                           : !prox_norm_from_tile(op, view(S), view(T)))
+                                   : ssd8 && !mfma;
       const double *integ = need_integ ? sqr_integral_for(c, S, ch) : nullptr;
       if (mfma) {
          size_t wb = crosscorr_mfma_work_bytes(precision, S.desc.Width, S.desc.Height, tw, th);

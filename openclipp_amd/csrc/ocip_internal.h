@@ -180,6 +180,9 @@ void launch_template_prep(hipStream_t s, const View &tpl, int channel, int pitch
 void launch_proximity(hipStream_t s, int op, const View &src, const View &tpl, void *dst,
                       int dst_step, const void *tstage, int tpitch, const double *tsums,
                       const double *integ, int channel);
+// The plain Norm ops on F32 sources whose template fits one LDS band form the window sums
+// of s^2 inside the tile kernel (launch_proximity with no integral)
+bool prox_norm_from_tile(int op, const View &src, const View &tpl);
 
 // MFMA cross-correlation (F32 1-channel source and template), precision 1..3.
 // The template's fragments and sum of squares (tsums[0]) are built from T in the call.

@@ -121,6 +121,69 @@ __device__ __forceinline__ void stage_tile_f32(float *lds, int ldsw, int nrows, 
    }
 }
 
+// Window sums of s^2 of a tile's outputs from the staged tile (F32 values of rows y0 - cy ..
+// y0 - cy + kTileRows + th - 2, columns x0 - cx .. x0 - cx + kWaves R + tw - 2, zero outside
+// the image), in FP64, for the Norm ops when the whole template fits one band.  Two passes
+// through LDS past the staged tile (tile_sqsum_lds bytes):
+//   h(rho, q)  = sum over u < tw of s(q + u, rho)^2    (rho < kTileRows + th - 1, q < kWaves R)
+//   SS(y, q)   = sum over v < th of h(y + v, q)         (y < kTileRows)
+// each as the difference of two running sums (lead - lag) of exact FP64 squares along its
+// row or column, started at the tile (a window of zeros gives exactly 0).  Returns SS of the
+// lane's row y = lane and columns colbase .. colbase + R - 1.  Every thread of the workgroup
+// takes part (two barriers).
+__host__ __device__ constexpr size_t tile_sqsum_offset(int rows, int ldsw)
+{
+   return ((size_t)rows * ldsw * sizeof(float) + 15) / 16 * 16;
+}
+size_t tile_sqsum_lds(int R, int th)
+{
+   return (size_t)(kTileRows + th - 1 + kTileRows) * kWaves * R * sizeof(double);
+}
+
+template <int R>
+__device__ __forceinline__ void tile_sqsums(float *lds, int ldsw, int tw, int th, int lane,
+                                            int colbase, double (&ss)[R])
+{
+   constexpr int C4 = kWaves * R;
+   const int nrows = kTileRows + th - 1;
+   double *hb = reinterpret_cast<double *>(reinterpret_cast<char *>(lds) +
+                                           tile_sqsum_offset(nrows, ldsw));
+   double *sb = hb + (size_t)nrows * C4;
+   // h: one task per staged row and R-column group
+   for (int t = threadIdx.x; t < nrows * kWaves; t += 256) {
+      const int rho = t / kWaves, q0 = (t - rho * kWaves) * R;
+      const float *row = lds + (size_t)rho * ldsw + q0;
+      double lead = 0.0, lag = 0.0;
+      for (int u = 0; u < tw; ++u) lead = fma((double)row[u], (double)row[u], lead);
+      double *hrow = hb + (size_t)rho * C4 + q0;
+      hrow[0] = lead;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+         lead = fma((double)row[tw + r - 1], (double)row[tw + r - 1], lead);
+         lag = fma((double)row[r - 1], (double)row[r - 1], lag);
+         hrow[r] = lead - lag;
+      }
+   }
+   __syncthreads();
+   // SS: 256 threads = C4 columns x (256 / C4) groups of consecutive output rows
+   {
+      constexpr int kGroups = 256 / C4, kRows = kTileRows / kGroups;
+      static_assert(256 % C4 == 0 && kTileRows % kGroups == 0, "row groups");
+      const int q = threadIdx.x % C4, y0 = (threadIdx.x / C4) * kRows;
+      double lead = 0.0, lag = 0.0;
+      for (int v = 0; v < th; ++v) lead += hb[(size_t)(y0 + v) * C4 + q];
+      sb[(size_t)y0 * C4 + q] = lead;
+      for (int y = y0 + 1; y < y0 + kRows; ++y) {
+         lead += hb[(size_t)(y + th - 1) * C4 + q];
+         lag += hb[(size_t)(y - 1) * C4 + q];
+         sb[(size_t)y * C4 + q] = lead - lag;
+      }
+   }
+   __syncthreads();
+#pragma unroll
+   for (int r = 0; r < R; ++r) ss[r] = sb[(size_t)lane * C4 + colbase + r];
+}
+
 template <int OP, int TYPE, int R>
 __global__ __launch_bounds__(256) void prox_tile_f32(const char *src, int W, int H, int sstep,
                                                       int nch, int ch,
@@ -165,17 +228,24 @@ __global__ __launch_bounds__(256) void prox_tile_f32(const char *src, int W, int
    }
 
    const int oy = y0 + lane;
-   if (oy >= H) return;
-   float *drow = reinterpret_cast<float *>(dst + (size_t)oy * dstep);
    if (OP == kSSDNorm || OP == kCCNorm) {
-      // all integral corners of the R outputs first, then the divisions
       const double tt = tsums[0];
       double ss[R];
+      if (integ) {
+         // all integral corners of the R outputs first, then the divisions
 #pragma unroll
-      for (int r = 0; r < R; ++r) ss[r] = window_sqsum(integ, W, H, x0 + colbase + r, oy, tw, th);
+         for (int r = 0; r < R; ++r) ss[r] = window_sqsum(integ, W, H, x0 + colbase + r, oy, tw, th);
+      } else {
+         // the whole template was staged in one band (the host passes no integral only
+         // then): the window sums of s^2 from the staged tile, in FP64
+         tile_sqsums<R>(lds, ldsw, tw, th, lane, colbase, ss);
+      }
+      if (oy >= H) return;
 #pragma unroll
       for (int r = 0; r < R; ++r) acc[r] = (float)((double)acc[r] / sqrt(ss[r] * tt));
    }
+   if (oy >= H) return;
+   float *drow = reinterpret_cast<float *>(dst + (size_t)oy * dstep);
 #pragma unroll
    for (int r = 0; r < R; ++r) {
       const int ox = x0 + colbase + r;
@@ -468,6 +538,21 @@ void launch_template_prep(hipStream_t s, const View &t, int channel, int pitch, 
    launch_check("template_prep");
 }
 
+// Output columns per lane of the tiled kernels
+static int tile_r(int W, int tw, int th) { return (W >= 2048 || tw * th >= 1024) ? 16 : 8; }
+
+bool prox_norm_from_tile(int op, const View &S, const View &T)
+{
+   if ((op != kSSDNorm && op != kCCNorm) || S.type != SImage::F32 || T.width <= 0 || T.height <= 0)
+      return false;
+   const int R = tile_r(S.width, T.width, T.height);
+   const int ldsw = lds_pitch(kWaves * R + T.width - 1);
+   const int rows_fit = kLdsBudget / (ldsw * 4) - (kTileRows - 1);
+   if (rows_fit < T.height) return false;   // one band
+   return tile_sqsum_offset(kTileRows + T.height - 1, ldsw) + tile_sqsum_lds(R, T.height) <=
+          (size_t)kLdsBudget;
+}
+
 template <int OP>
 static void launch_op(hipStream_t s, const View &S, const View &T, void *dst, int dstep,
                       const void *tstage, int tpitch, const double *tsums, const double *integ,
@@ -477,7 +562,7 @@ static void launch_op(hipStream_t s, const View &S, const View &T, void *dst, in
    const char *src = static_cast<const char *>(S.ptr);
    const bool small_int = S.type == SImage::U8 || S.type == SImage::S8 ||
                           S.type == SImage::U16 || S.type == SImage::S16;
-   const int R = (W >= 2048 || tw * th >= 1024) ? 16 : 8;
+   const int R = tile_r(W, tw, th);
    const int ncols = kWaves * R + tw - 1;
    const int ldsw = lds_pitch(ncols);
    const int rows_fit = kLdsBudget / (ldsw * 4) - (kTileRows - 1);
@@ -508,8 +593,13 @@ static void launch_op(hipStream_t s, const View &S, const View &T, void *dst, in
    }
 
    const int vb = min(th, rows_fit);
-   const size_t lds = (size_t)(kTileRows + vb - 1) * ldsw * 4;
+   size_t lds = (size_t)(kTileRows + vb - 1) * ldsw * 4;
    dim3 grid((W + kWaves * R - 1) / (kWaves * R), (H + kTileRows - 1) / kTileRows);
+   // Norm ops on F32 sources with no integral: window sums from the staged tile
+   if ((OP == kSSDNorm || OP == kCCNorm) && !integ) {
+      if (!prox_norm_from_tile(OP, S, T)) fail(CL_INVALID_VALUE, "integral required");
+      lds = tile_sqsum_offset(kTileRows + vb - 1, ldsw) + tile_sqsum_lds(R, th);
+   }
 
 #define OCIP_TILE_F32(TT_, R_)                                                                 \
    prox_tile_f32<OP, TT_, R_><<<grid.x * grid.y, 256, lds, s>>>(                                         \
