@@ -53,17 +53,28 @@ __device__ __forceinline__ double block_sum_d(double v, double *red)
 // relative error stays near 2^-53 times (tile energy / window energy).
 constexpr int kBxRows = 32, kBxCols = 64;
 
+// Per-image arguments of the batched kernels (blockIdx.y = image of a chunk): a pointer and a
+// row pitch per image, passed by value among the kernel arguments.
+struct ZPtrs {
+   const char *p[kSplitImages];
+   int step[kSplitImages];
+};
+
 size_t box_f64_lds(int tw, int th)
 {
    const size_t ncols = kBxCols - 1 + tw, nrows = kBxRows - 1 + th;
    return (size_t)kBxRows * ncols * sizeof(double) + nrows * ncols * sizeof(float);
 }
 
-__global__ __launch_bounds__(kThreads) void box_f64(const char *m, int mstep, int W, int H,
-                                                     int tw, int th, int ax, int ay, int sqr,
-                                                     double *out, int tiles_x)
+__global__ __launch_bounds__(kThreads) void box_f64(ZPtrs maps, int W, int H, int tw, int th,
+                                                     int ax, int ay, int sqr, double *outs,
+                                                     size_t ostride, int tiles_x)
 {
    extern __shared__ double bx_lds[];
+   // image blockIdx.y: its map, and its output at outs + blockIdx.y * ostride
+   const char *m = maps.p[blockIdx.y];
+   const int mstep = maps.step[blockIdx.y];
+   double *out = outs + blockIdx.y * ostride;
    const int ncols = kBxCols - 1 + tw, nrows = kBxRows - 1 + th;
    double *vs = bx_lds;                                            // kBxRows x ncols
    float *ms = reinterpret_cast<float *>(vs + kBxRows * ncols);   // nrows x ncols
@@ -118,8 +129,9 @@ __global__ __launch_bounds__(kThreads) void box_f64(const char *m, int mstep, in
    }
 }
 
-void launch_box_f64(hipStream_t s, const char *m, int mstep, int W, int H, int tw, int th,
-                    int ax, int ay, bool sqr, double *out)
+// Window sums of the maps of n images (maps.p[z], z < n) into out + z * ostride
+void launch_box_f64(hipStream_t s, const ZPtrs &maps, int n, int W, int H, int tw, int th,
+                    int ax, int ay, bool sqr, double *out, size_t ostride)
 {
    const int tiles_x = (W + kBxCols - 1) / kBxCols;
    const int tiles = tiles_x * ((H + kBxRows - 1) / kBxRows);
@@ -128,7 +140,8 @@ void launch_box_f64(hipStream_t s, const char *m, int mstep, int W, int H, int t
       hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(box_f64),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                 "lds attribute");
-   box_f64<<<tiles, kThreads, lds, s>>>(m, mstep, W, H, tw, th, ax, ay, sqr ? 1 : 0, out, tiles_x);
+   box_f64<<<dim3(tiles, n), kThreads, lds, s>>>(maps, W, H, tw, th, ax, ay, sqr ? 1 : 0, out,
+                                                ostride, tiles_x);
    launch_check("box_f64");
 }
 
@@ -139,16 +152,27 @@ void launch_box_f64(hipStream_t s, const char *m, int mstep, int W, int H, int t
 //   Eb = 2 G2 - G1 N / SS       (dS: s times its flipped window sums)
 // each formed in FP64 and rounded once, and per-block partial sums of G1 N (sncc) and of G2
 // (sssd).  SS comes from the dense window-sum map `box` (box_f64), or when that is null from
-// the exclusive FP64 integral `integ`.
-__global__ __launch_bounds__(kThreads) void grad_maps(const float *C, int cstep, const char *G1,
-                                                       int g1step, const char *G2, int g2step,
-                                                       const double *__restrict__ box,
+// the exclusive FP64 integral `integ`.  Image z = blockIdx.y: C, box, M and Eb at z * zstride
+// elements from their bases (W x H, rows W apart), G1 / G2 from the tables (has1 / has2: the
+// term present), partial sums at sncc / sssd + z * gridDim.x.
+__global__ __launch_bounds__(kThreads) void grad_maps(const float *Cs, ZPtrs g1s, ZPtrs g2s,
+                                                       int has1, int has2,
+                                                       const double *__restrict__ boxes,
                                                        const double *__restrict__ integ,
                                                        const double *__restrict__ tsums, int W,
-                                                       int H, int tw, int th, float *M, float *Eb,
-                                                       double *sncc, double *sssd)
+                                                       int H, int tw, int th, float *Ms,
+                                                       float *Ebs, size_t zstride,
+                                                       double *snccs, double *sssds)
 {
    __shared__ double red[4];
+   const size_t zo = blockIdx.y * zstride;
+   const float *C = Cs + zo;
+   const double *box = boxes ? boxes + zo : nullptr;
+   float *M = Ms + zo, *Eb = Ebs + zo;
+   const char *G1 = has1 ? g1s.p[blockIdx.y] : nullptr, *G2 = has2 ? g2s.p[blockIdx.y] : nullptr;
+   const int g1step = g1s.step[blockIdx.y], g2step = g2s.step[blockIdx.y];
+   double *sncc = snccs + (size_t)blockIdx.y * gridDim.x, *sssd = sssds + (size_t)blockIdx.y * gridDim.x;
+   const int cstep = W * (int)sizeof(float);
    const double tt = tsums[0];
    const int cx = tw / 2, cy = th / 2;
    double a1 = 0, a2 = 0;
@@ -291,10 +315,16 @@ __device__ __forceinline__ tg_f32x16 tg_mma(tg_u16x8 a, tg_u16x8 b, tg_f32x16 c)
                                                   __builtin_bit_cast(tg_bf16x8, b), c, 0, 0, 0);
 }
 
-__global__ __launch_bounds__(256) void tgrad_mfma(const char *M, int mstep, const float *S,
-                                                   int sstep, int W, int H, int tw, int th,
-                                                   int tasks_x, float *partial)
+__global__ __launch_bounds__(256) void tgrad_mfma(const char *Ms, int mstep, size_t mzstride,
+                                                   ZPtrs srcs, int W, int H, int tw, int th,
+                                                   int tasks_x, float *partials, size_t pzstride)
 {
+   // image blockIdx.y: its map at Ms + blockIdx.y * mzstride bytes, its source from the table,
+   // its partials at partials + blockIdx.y * pzstride
+   const char *M = Ms + blockIdx.y * mzstride;
+   const float *S = reinterpret_cast<const float *>(srcs.p[blockIdx.y]);
+   const int sstep = srcs.step[blockIdx.y];
+   float *partial = partials + blockIdx.y * pzstride;
    // planes: source hi, source lo (kTgPRows x kTgCols); map hi, lo, each in two copies
    // (kTgRows x kTgMCols)
    __shared__ unsigned short ps[2][kTgPRows * kTgCols];
@@ -399,50 +429,62 @@ __global__ __launch_bounds__(256) void tgrad_mfma(const char *M, int mstep, cons
    }
 }
 
-// dT(u, v) = sum_b partial[b][tap] + c t(u, v), c = 2 (sum G2) - (sum G1 N) / TT, the sums
-// from the per-block partials sssd (nssd of them) and sncc (nncc; either count may be 0).
-__global__ __launch_bounds__(kThreads) void grad_template(const float *partial, int nblocks,
+// The template-gradient partials are reduced in two stages.  First stage:
+// red[g][tap] = sum of partial[b][tap] over the kRedBlocks consecutive blocks b of group g, in
+// FP64; 64 taps per workgroup (lane = tap), the four waves take blocks b = wave (mod 4) eight
+// at a time (loads in flight) in order, then lane's four sums are added in wave order.
+constexpr int kRedBlocks = 128;
+
+__global__ __launch_bounds__(kThreads) void grad_partials(const float *partial, int nblocks,
+                                                           int ntaps, double *red)
+{
+   __shared__ double part[4][64];
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   const int tap = blockIdx.x * 64 + lane;
+   const int b0 = blockIdx.y * kRedBlocks, b1 = min(nblocks, b0 + kRedBlocks);
+   double r = 0;
+   if (tap < ntaps) {
+      int k = b0 + wave;
+      for (; k + 28 < b1; k += 32) {
+         float q[8];
+#pragma unroll
+         for (int j = 0; j < 8; ++j) q[j] = partial[(size_t)(k + 4 * j) * ntaps + tap];
+#pragma unroll
+         for (int j = 0; j < 8; ++j) r += q[j];
+      }
+      for (; k < b1; k += 4) r += partial[(size_t)k * ntaps + tap];
+   }
+   part[wave][lane] = r;
+   __syncthreads();
+   if (wave == 0 && tap < ntaps)
+      red[(size_t)blockIdx.y * ntaps + tap] =
+          ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+}
+
+// Second stage: dT(u, v) = sum_g red[g][tap] + c t(u, v) (groups in order), c = 2 (sum G2) -
+// (sum G1 N) / TT, the sums from the per-block partials sssd (nssd of them) and sncc (nncc;
+// either count may be 0).
+__global__ __launch_bounds__(kThreads) void grad_template(const double *red, int ngroups,
                                                            const double *sncc, int nncc,
                                                            const double *sssd, int nssd,
                                                            const float *tstage, int tpitch,
                                                            const double *tsums, int tw, int th,
                                                            char *dT, int dtstep)
 {
-   __shared__ double red[4];
+   __shared__ double sred[4];
    double s1 = 0, s2 = 0;
    for (int k = threadIdx.x; k < nncc; k += kThreads) s1 += sncc[k];
    for (int k = threadIdx.x; k < nssd; k += kThreads) s2 += sssd[k];
-   s1 = block_sum_d(s1, red);
-   s2 = block_sum_d(s2, red);
+   s1 = block_sum_d(s1, sred);
+   s2 = block_sum_d(s2, sred);
    const double cterm = 2.0 * s2 - s1 / tsums[0];
-   // 64 taps per workgroup (lane = tap); the four waves sum the blocks k = wave (mod 4),
-   // then lane's four sums are added in wave order
-   __shared__ double part[4][64];
-   const int ntaps = tw * th, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-   for (int t0 = blockIdx.x * 64; t0 < ntaps; t0 += gridDim.x * 64) {
-      const int tap = t0 + lane;
+   const int ntaps = tw * th;
+   for (int tap = blockIdx.x * kThreads + threadIdx.x; tap < ntaps; tap += gridDim.x * kThreads) {
       double r = 0;
-      if (tap < ntaps) {
-         // blocks k = wave (mod 4): eight at a time (loads in flight), added in order
-         int k = wave;
-         for (; k + 28 < nblocks; k += 32) {
-            float q[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) q[j] = partial[(size_t)(k + 4 * j) * ntaps + tap];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) r += q[j];
-         }
-         for (; k < nblocks; k += 4) r += partial[(size_t)k * ntaps + tap];
-      }
-      part[wave][lane] = r;
-      __syncthreads();
-      if (wave == 0 && tap < ntaps) {
-         r = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
-         const int v = tap / tw, u = tap - v * tw;
-         const double t = tstage[(size_t)v * tpitch + u];
-         reinterpret_cast<float *>(dT + (size_t)v * dtstep)[u] = (float)(r + cterm * t);
-      }
-      __syncthreads();
+      for (int g = 0; g < ngroups; ++g) r += red[(size_t)g * ntaps + tap];
+      const int v = tap / tw, u = tap - v * tw;
+      const double t = tstage[(size_t)v * tpitch + u];
+      reinterpret_cast<float *>(dT + (size_t)v * dtstep)[u] = (float)(r + cterm * t);
    }
 }
 
@@ -461,13 +503,22 @@ __global__ void flip_template(const float *tstage, int tpitch, int tw, int th, i
 }
 
 // dS(j, i) = F(j, i) + s(j, i) * (flipped window sum of a map: the dense window-sum map `bm`
-// (box_f64), or when that is null from the map's integral I), in FP64, rounded once
-__global__ __launch_bounds__(kThreads) void grad_source(const float *F, const float *S, int sstep,
-                                                         const double *__restrict__ bm,
-                                                         const double *__restrict__ I, int W,
-                                                         int H, int tw, int th, char *dS,
-                                                         int dsstep)
+// (box_f64), or when that is null from the map's integral I), in FP64, rounded once.  Image
+// z = blockIdx.y: F and bm at z * zstride elements from their bases, source and dS from the
+// tables (a null dS: nothing to write).
+__global__ __launch_bounds__(kThreads) void grad_source(const float *Fs, ZPtrs srcs,
+                                                         const double *__restrict__ bms,
+                                                         const double *__restrict__ I,
+                                                         size_t zstride, int W, int H, int tw,
+                                                         int th, ZPtrs dss)
 {
+   char *dS = const_cast<char *>(dss.p[blockIdx.y]);
+   if (!dS) return;
+   const int dsstep = dss.step[blockIdx.y];
+   const float *F = Fs + blockIdx.y * zstride;
+   const double *bm = bms ? bms + blockIdx.y * zstride : nullptr;
+   const float *S = reinterpret_cast<const float *>(srcs.p[blockIdx.y]);
+   const int sstep = srcs.step[blockIdx.y];
    const int cx = tw / 2, cy = th / 2;
    for (long long k = (long long)blockIdx.x * kThreads + threadIdx.x; k < (long long)W * H;
         k += (long long)gridDim.x * kThreads) {
@@ -486,7 +537,7 @@ int grid_for(long long n) { return (int)std::min<long long>(1024, (n + kThreads 
 constexpr int kChunkPixels = 16 << 20;
 
 struct BwLayout {
-   size_t F, M, E, I, partial, scal, tflip, fragT, fragF, sums, total;
+   size_t F, M, E, I, partial, red, scal, tflip, fragT, fragF, sums, total;
    int nb;        // template-gradient partial blocks per image
    int zc;        // images per chunk
    int twf, thf;  // flipped template, zero-padded to odd sizes
@@ -522,8 +573,11 @@ BwLayout bw_layout(int n, int W, int H, int tw, int th)
    L.F = o;       o = align_up(o + maps);
    L.M = o;       o = align_up(o + maps);
    L.E = o;       o = align_up(o + maps);
-   L.I = o;       o = align_up(o + excl_bytes(W, H));    // a dense W x H FP64 map fits too
+   // the chunk's dense W x H FP64 window-sum maps, or one exclusive integral
+   L.I = o;       o = align_up(o + std::max(excl_bytes(W, H), (size_t)L.zc * px * sizeof(double)));
    L.partial = o; o = align_up(o + (size_t)n * L.nb * tw * th * sizeof(float));
+   L.red = o;     o = align_up(o + (size_t)((n * L.nb + kRedBlocks - 1) / kRedBlocks) * tw * th *
+                                   sizeof(double));
    L.scal = o;    o = align_up(o + (size_t)2 * n * 1024 * sizeof(double));
    L.tflip = o;   o = align_up(o + (size_t)L.twf * L.thf * sizeof(float));
    L.fragT = o;   o = align_up(o + crosscorr_split_frag_bytes(tw, th));
@@ -586,8 +640,29 @@ void launch_backward_batch(hipStream_t s, int n, const View *S, const View *G1, 
    const float *src[kSplitImages];
    float *dst[kSplitImages];
    int step[kSplitImages];
+   auto one = [](const void *p, int st) {
+      ZPtrs z{};
+      z.p[0] = static_cast<const char *>(p);
+      z.step[0] = st;
+      return z;
+   };
    for (int c0 = 0; c0 < n; c0 += L.zc) {
       const int m = std::min(L.zc, n - c0);
+      // the chunk's tables: sources, gradient maps, dS outputs, and the internal maps E
+      ZPtrs zs{}, zg1{}, zg2{}, zds{}, ze{};
+      bool chunk_ds = false;
+      for (int k = 0; k < m; ++k) {
+         const int i = c0 + k;
+         zs.p[k] = static_cast<const char *>(S[i].ptr);
+         zs.step[k] = S[i].step;
+         if (G1) { zg1.p[k] = static_cast<const char *>(G1[i].ptr); zg1.step[k] = G1[i].step; }
+         if (G2) { zg2.p[k] = static_cast<const char *>(G2[i].ptr); zg2.step[k] = G2[i].step; }
+         zds.p[k] = dS[i];
+         zds.step[k] = dS[i] ? dsstep[i] : 0;
+         ze.p[k] = reinterpret_cast<const char *>(E0 + (size_t)k * px);
+         ze.step[k] = W * 4;
+         chunk_ds = chunk_ds || dS[i];
+      }
       if (G1) {
          // the chunk's forward CrossCorr maps
          for (int k = 0; k < m; ++k) {
@@ -597,31 +672,33 @@ void launch_backward_batch(hipStream_t s, int n, const View *S, const View *G1, 
          }
          launch_crosscorr_split(s, m, src, step, dst, W * 4, W, H, tw, th, w + L.fragT, sums);
       }
-      for (int k = 0; k < m; ++k) {
-         const int i = c0 + k;
-         const float *Sp = static_cast<const float *>(S[i].ptr);
-         float *F = F0 + (size_t)k * px, *M = M0 + (size_t)k * px, *E = E0 + (size_t)k * px;
-         if (G1) {
-            // the windows' sums of s^2
-            if (use_box)
-               launch_box_f64(s, static_cast<const char *>(S[i].ptr), S[i].step, W, H, tw, th,
-                              tw / 2, th / 2, true, I);
-            else
-               launch_integral(s, S[i], 0, true, I, 0, -1, aux);
-         }
-         grad_maps<<<nscal, kThreads, 0, s>>>(
-             F, W * 4, G1 ? static_cast<const char *>(G1[i].ptr) : nullptr, G1 ? G1[i].step : 0,
-             G2 ? static_cast<const char *>(G2[i].ptr) : nullptr, G2 ? G2[i].step : 0,
-             use_box ? I : nullptr, use_box ? nullptr : I, tsums, W, H, tw, th, M, E,
-             sncc + (size_t)i * nscal, sssd + (size_t)i * nscal);
+      // the maps M and Eb (and partial sums) after the windows' sums of s^2
+      if (use_box) {
+         if (G1) launch_box_f64(s, zs, m, W, H, tw, th, tw / 2, th / 2, true, I, px);
+         grad_maps<<<dim3(nscal, m), kThreads, 0, s>>>(
+             F0, zg1, zg2, G1 ? 1 : 0, G2 ? 1 : 0, G1 ? I : nullptr, nullptr, tsums, W, H, tw,
+             th, M0, E0, px, sncc + (size_t)c0 * nscal, sssd + (size_t)c0 * nscal);
          launch_check("grad_maps");
-         if (!dT) continue;
-         const char *Mc = reinterpret_cast<const char *>(M);
-         float *pi = partial + (size_t)i * L.nb * taps;
+      } else {
+         for (int k = 0; k < m; ++k) {
+            const int i = c0 + k;
+            if (G1) launch_integral(s, S[i], 0, true, I, 0, -1, aux);
+            grad_maps<<<dim3(nscal, 1), kThreads, 0, s>>>(
+                F0 + (size_t)k * px, G1 ? one(G1[i].ptr, G1[i].step) : ZPtrs{},
+                G2 ? one(G2[i].ptr, G2[i].step) : ZPtrs{}, G1 ? 1 : 0, G2 ? 1 : 0, nullptr,
+                G1 ? I : nullptr, tsums, W, H, tw, th, M0 + (size_t)k * px, E0 + (size_t)k * px, 0,
+                sncc + (size_t)i * nscal, sssd + (size_t)i * nscal);
+            launch_check("grad_maps");
+         }
+      }
+      if (dT) {
+         float *pc = partial + (size_t)c0 * L.nb * taps;
          if (tgrad_on_mfma(tw, th)) {
             // the contraction on MFMA (split BF16); the FP64 reduction follows the batch
             const int tasks_x = (W + kTgCols - 1) / kTgCols;
-            tgrad_mfma<<<L.nb, kThreads, 0, s>>>(Mc, W * 4, Sp, S[i].step, W, H, tw, th, tasks_x, pi);
+            tgrad_mfma<<<dim3(L.nb, m), kThreads, 0, s>>>(
+                reinterpret_cast<const char *>(M0), W * 4, px * sizeof(float), zs, W, H, tw, th,
+                tasks_x, pc, (size_t)L.nb * taps);
             launch_check("tgrad_mfma");
          } else {
             const int tile = wc_tile(tw, th);
@@ -630,53 +707,60 @@ void launch_backward_batch(hipStream_t s, int n, const View *S, const View *G1, 
             int ldsw = (tile + tw - 1 + 4 + 3) / 4 * 4;
             while (ldsw % 32 != 4) ldsw += 4;
             const size_t lds = (size_t)tile * tile * 4 + (size_t)(tile + th - 1) * ldsw * 4;
+            if (lds > 160 * 1024) fail(CL_INVALID_VALUE, "template too large for the gradient");
             dim3 grid((W + tile - 1) / tile, (H + tile - 1) / tile);
-            if (tile == 64) {
-               hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(weight_corr<64>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), "attr");
-               weight_corr<64><<<grid, kThreads, lds, s>>>(Mc, W * 4, Sp, S[i].step, W, H, tw, th, pi, ldsw);
-            } else {
-               if (lds > 160 * 1024) fail(CL_INVALID_VALUE, "template too large for the gradient");
-               hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(weight_corr<32>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), "attr");
-               weight_corr<32><<<grid, kThreads, lds, s>>>(Mc, W * 4, Sp, S[i].step, W, H, tw, th, pi, ldsw);
+            for (int k = 0; k < m; ++k) {
+               const char *Mc = reinterpret_cast<const char *>(M0 + (size_t)k * px);
+               const float *Sp = static_cast<const float *>(S[c0 + k].ptr);
+               float *pi = pc + (size_t)k * L.nb * taps;
+               if (tile == 64) {
+                  hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(weight_corr<64>),
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), "attr");
+                  weight_corr<64><<<grid, kThreads, lds, s>>>(Mc, W * 4, Sp, S[c0 + k].step, W, H, tw, th, pi, ldsw);
+               } else {
+                  hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(weight_corr<32>),
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), "attr");
+                  weight_corr<32><<<grid, kThreads, lds, s>>>(Mc, W * 4, Sp, S[c0 + k].step, W, H, tw, th, pi, ldsw);
+               }
+               launch_check("weight_corr");
             }
-            launch_check("weight_corr");
          }
       }
+      if (!chunk_ds) continue;
       // the chunk's correlations of M with the flipped template, into F
-      int md = 0;
       for (int k = 0; k < m; ++k) {
-         if (!dS[c0 + k]) continue;
-         src[md] = M0 + (size_t)k * px;
-         step[md] = W * 4;
-         dst[md] = F0 + (size_t)k * px;
-         ++md;
+         src[k] = M0 + (size_t)k * px;
+         step[k] = W * 4;
+         dst[k] = F0 + (size_t)k * px;
       }
-      if (md) launch_crosscorr_split(s, md, src, step, dst, W * 4, W, H, L.twf, L.thf, w + L.fragF,
-                                     sums + 4);
-      for (int k = 0; k < m; ++k) {
-         const int i = c0 + k;
-         if (!dS[i]) continue;
-         const float *Sp = static_cast<const float *>(S[i].ptr);
-         float *F = F0 + (size_t)k * px, *E = E0 + (size_t)k * px;
-         // flipped window sums of Eb (window anchored at tw - 1 - cx, th - 1 - cy)
-         if (use_box) {
-            launch_box_f64(s, reinterpret_cast<const char *>(E), W * 4, W, H, tw, th,
-                           tw - 1 - tw / 2, th - 1 - th / 2, false, I);
-         } else {
-            const View ev{E, W, H, W * 4, 1, SImage::F32};
-            launch_integral(s, ev, 0, false, I, 0, -1, aux);
-         }
-         grad_source<<<grid_for((long long)px), kThreads, 0, s>>>(
-             F, Sp, S[i].step, use_box ? I : nullptr, use_box ? nullptr : I, W, H, tw, th, dS[i],
-             dsstep[i]);
+      launch_crosscorr_split(s, m, src, step, dst, W * 4, W, H, L.twf, L.thf, w + L.fragF, sums + 4);
+      // dS from F, s and the flipped window sums of Eb (window anchored at tw - 1 - cx,
+      // th - 1 - cy)
+      if (use_box) {
+         launch_box_f64(s, ze, m, W, H, tw, th, tw - 1 - tw / 2, th - 1 - th / 2, false, I, px);
+         grad_source<<<dim3(grid_for((long long)px), m), kThreads, 0, s>>>(
+             F0, zs, I, nullptr, px, W, H, tw, th, zds);
          launch_check("grad_source");
+      } else {
+         for (int k = 0; k < m; ++k) {
+            const int i = c0 + k;
+            if (!dS[i]) continue;
+            const View ev{E0 + (size_t)k * px, W, H, W * 4, 1, SImage::F32};
+            launch_integral(s, ev, 0, false, I, 0, -1, aux);
+            grad_source<<<dim3(grid_for((long long)px), 1), kThreads, 0, s>>>(
+                F0 + (size_t)k * px, one(S[i].ptr, S[i].step), nullptr, I, 0, W, H, tw, th,
+                one(dS[i], dsstep[i]));
+            launch_check("grad_source");
+         }
       }
    }
    if (dT) {
-      grad_template<<<(taps + 63) / 64, kThreads, 0, s>>>(
-          partial, n * L.nb, sncc, G1 ? n * nscal : 0, sssd, G2 ? n * nscal : 0, tstage, tpitch,
+      const int nblocks = n * L.nb, ngroups = (nblocks + kRedBlocks - 1) / kRedBlocks;
+      double *red = reinterpret_cast<double *>(w + L.red);
+      grad_partials<<<dim3((taps + 63) / 64, ngroups), kThreads, 0, s>>>(partial, nblocks, taps, red);
+      launch_check("grad_partials");
+      grad_template<<<(taps + kThreads - 1) / kThreads, kThreads, 0, s>>>(
+          red, ngroups, sncc, G1 ? n * nscal : 0, sssd, G2 ? n * nscal : 0, tstage, tpitch,
           tsums, tw, th, dT, dtstep);
       launch_check("grad_template");
    }

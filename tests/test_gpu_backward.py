@@ -356,6 +356,27 @@ def test_batch_backward_chunks_and_missing_outputs(ocip):
     assert np.max(np.abs(dT.read() - want_t)) <= 5e-5 * np.abs(want_t).max()
 
 
+def test_batch_backward_large_template_against_fp64_reference(ocip):
+    """A 130 x 120 template, whose window sums come from FP64 integrals (too large for the
+    box-sum tile) and whose template gradient from the FP32 contraction: two images in one
+    ocipProximityBackwardBatch call, every dS_i and the summed dT within 2e-4 of the largest
+    component of the FP64 reference gradient."""
+    rng = np.random.default_rng(130)
+    n, H, W, th, tw = 2, 200, 260, 120, 130
+    srcs = [rng.random((H, W)).astype(np.float32) for _ in range(n)]
+    tpl = rng.normal(0, 1, (th, tw)).astype(np.float32)
+    g1 = [rng.normal(0, 1, (H, W)).astype(np.float32) for _ in range(n)]
+    g2 = [rng.normal(0, 1, (H, W)).astype(np.float32) for _ in range(n)]
+    say("batch backward large template", tw, th)
+    ds, dt = _batch_backward(ocip, srcs, tpl, g1, g2)
+    want_t = np.zeros(tpl.shape)
+    for k in range(n):
+        want_s, wt = ref_grad_fft(srcs[k], tpl, g1[k], g2[k])
+        want_t += wt
+        assert np.max(np.abs(ds[k] - want_s)) / np.abs(want_s).max() <= 2e-4, k
+    assert np.max(np.abs(dt - want_t)) / np.abs(want_t).max() <= 2e-4
+
+
 def test_batch_backward_diff_1024_against_fp64_reference(ocip):
     """The whole 16 x 1024x1024 / 31x31 workload in one ocipProximityBackwardBatch call (both
     ops): every dS_i and the summed dT within 2e-4 of the largest component of the FP64
