@@ -81,11 +81,12 @@ __global__ __launch_bounds__(kThreads) void box_f64(ZPtrs maps, int W, int H, in
    const int x0 = (blockIdx.x % tiles_x) * kBxCols, y0 = (blockIdx.x / tiles_x) * kBxRows;
    const int c0 = x0 - ax, r0 = y0 - ay;
    const int total = nrows * ncols;
-   // staging: eight loads of each thread in flight at a time
-   for (int k0 = threadIdx.x; k0 < total; k0 += 8 * kThreads) {
-      float f[8];
+   // staging: up to 32 loads of each thread in flight at a time (a 31 x 31 template's tile
+   // takes 23 per thread)
+   for (int k0 = threadIdx.x; k0 < total; k0 += 32 * kThreads) {
+      float f[32];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int j = 0; j < 32; ++j) {
          const int k = k0 + j * kThreads;
          const int r = k / ncols, c = k - r * ncols;
          const int y = r0 + r, x = c0 + c;
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(kThreads) void box_f64(ZPtrs maps, int W, int H, in
                     : 0.f;
       }
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int j = 0; j < 32; ++j) {
          const int k = k0 + j * kThreads;
          if (k < total) ms[k] = f[j];
       }
@@ -292,7 +293,7 @@ __global__ __launch_bounds__(kThreads) void weight_corr(const char *M, int mstep
 // Wave w takes the task's rows y0 + w, y0 + w + 4, ...; the four waves' 32 x 32 sums are
 // added in wave order and the task writes one partial per tap, which grad_template reduces in
 // FP64.
-constexpr int kTgRows = 16, kTgCols = 128;
+constexpr int kTgRows = 32, kTgCols = 128;
 constexpr int kTgPRows = kTgRows + 31;       // staged source rows
 constexpr int kTgMCols = kTgCols + 32;       // staged map columns (31 to the left, 1 spare)
 
@@ -461,27 +462,57 @@ __global__ __launch_bounds__(kThreads) void grad_partials(const float *partial, 
           ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
 }
 
-// Second stage: dT(u, v) = sum_g red[g][tap] + c t(u, v) (groups in order), c = 2 (sum G2) -
-// (sum G1 N) / TT, the sums from the per-block partials sssd (nssd of them) and sncc (nncc;
-// either count may be 0).
-__global__ __launch_bounds__(kThreads) void grad_template(const double *red, int ngroups,
-                                                           const double *sncc, int nncc,
-                                                           const double *sssd, int nssd,
-                                                           const float *tstage, int tpitch,
-                                                           const double *tsums, int tw, int th,
-                                                           char *dT, int dtstep)
+// The scalar sums of the per-block partials (n of them each, either n may be 0): out[0] =
+// sum of sncc, out[1] = sum of sssd, in FP64; one workgroup, each thread's share eight loads
+// at a time, then the workgroup sums in wave order.
+__global__ __launch_bounds__(kThreads) void grad_scalars(const double *sncc, int nncc,
+                                                          const double *sssd, int nssd,
+                                                          double *out)
 {
-   __shared__ double sred[4];
-   double s1 = 0, s2 = 0;
-   for (int k = threadIdx.x; k < nncc; k += kThreads) s1 += sncc[k];
-   for (int k = threadIdx.x; k < nssd; k += kThreads) s2 += sssd[k];
-   s1 = block_sum_d(s1, sred);
-   s2 = block_sum_d(s2, sred);
-   const double cterm = 2.0 * s2 - s1 / tsums[0];
-   const int ntaps = tw * th;
-   for (int tap = blockIdx.x * kThreads + threadIdx.x; tap < ntaps; tap += gridDim.x * kThreads) {
-      double r = 0;
-      for (int g = 0; g < ngroups; ++g) r += red[(size_t)g * ntaps + tap];
+   __shared__ double red[4];
+   double s[2] = {0.0, 0.0};
+#pragma unroll
+   for (int which = 0; which < 2; ++which) {
+      const double *p = which ? sssd : sncc;
+      const int n = which ? nssd : nncc;
+      int k = threadIdx.x;
+      for (; k + 7 * kThreads < n; k += 8 * kThreads) {
+         double q[8];
+#pragma unroll
+         for (int j = 0; j < 8; ++j) q[j] = p[k + j * kThreads];
+#pragma unroll
+         for (int j = 0; j < 8; ++j) s[which] += q[j];
+      }
+      for (; k < n; k += kThreads) s[which] += p[k];
+   }
+   const double a = block_sum_d(s[0], red);
+   const double b = block_sum_d(s[1], red);
+   if (threadIdx.x == 0) {
+      out[0] = a;
+      out[1] = b;
+   }
+}
+
+// Second stage: dT(u, v) = sum_g red[g][tap] + c t(u, v) (groups in order), c = 2 (sum G2) -
+// (sum G1 N) / TT from the scalar sums sc[0] = sum G1 N, sc[1] = sum G2.  64 taps per
+// workgroup (lane = tap); the four waves take groups g = wave (mod 4) in order, then lane's
+// four sums are added in wave order.
+__global__ __launch_bounds__(kThreads) void grad_template(const double *red, int ngroups,
+                                                           const double *sc, const float *tstage,
+                                                           int tpitch, const double *tsums,
+                                                           int tw, int th, char *dT, int dtstep)
+{
+   __shared__ double part[4][64];
+   const double cterm = 2.0 * sc[1] - sc[0] / tsums[0];
+   const int ntaps = tw * th, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   const int tap = blockIdx.x * 64 + lane;
+   double r = 0;
+   if (tap < ntaps)
+      for (int g = wave; g < ngroups; g += 4) r += red[(size_t)g * ntaps + tap];
+   part[wave][lane] = r;
+   __syncthreads();
+   if (wave == 0 && tap < ntaps) {
+      r = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
       const int v = tap / tw, u = tap - v * tw;
       const double t = tstage[(size_t)v * tpitch + u];
       reinterpret_cast<float *>(dT + (size_t)v * dtstep)[u] = (float)(r + cterm * t);
@@ -759,9 +790,11 @@ void launch_backward_batch(hipStream_t s, int n, const View *S, const View *G1, 
       double *red = reinterpret_cast<double *>(w + L.red);
       grad_partials<<<dim3((taps + 63) / 64, ngroups), kThreads, 0, s>>>(partial, nblocks, taps, red);
       launch_check("grad_partials");
-      grad_template<<<(taps + kThreads - 1) / kThreads, kThreads, 0, s>>>(
-          red, ngroups, sncc, G1 ? n * nscal : 0, sssd, G2 ? n * nscal : 0, tstage, tpitch,
-          tsums, tw, th, dT, dtstep);
+      double *sc = sums + 6;
+      grad_scalars<<<1, kThreads, 0, s>>>(sncc, G1 ? n * nscal : 0, sssd, G2 ? n * nscal : 0, sc);
+      launch_check("grad_scalars");
+      grad_template<<<(taps + 63) / 64, kThreads, 0, s>>>(red, ngroups, sc, tstage, tpitch,
+                                                         tsums, tw, th, dT, dtstep);
       launch_check("grad_template");
    }
 }
