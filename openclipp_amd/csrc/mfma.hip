@@ -353,15 +353,15 @@ struct Geometry {
    size_t lds;
 };
 
-// LDS rows of lcols_for(KC, NT) 16-bit values, pitch shifted by 16 bytes per row modulo 128
-// so that the 32 rows read by one A fragment start on different banks; as many template rows
-// per band as fit in 160 KiB.
+// LDS rows of lcols_for(KC, NT) 16-bit values at a pitch that is an odd multiple of 16 bytes,
+// so that any 8 consecutive rows (the 8 lanes of one 128-byte phase of an A-fragment read)
+// start on different 16-byte bank groups; as many template rows per band as fit in 160 KiB.
 Geometry geometry(int precision, int KC, int NT, int th_max)
 {
    Geometry g;
    const int lcols = mfma::lcols_for(KC, NT);
    g.ldsp = (lcols + 7) / 8 * 8;
-   while ((g.ldsp * 2) % 128 != 16) g.ldsp += 8;
+   while ((g.ldsp * 2) % 32 != 16) g.ldsp += 8;
    const long rows_fit = (long)mfma::kLdsMax / ((long)planes_for(precision) * g.ldsp * 2);
    g.vb = (int)std::min<long>(th_max, rows_fit - (mfma::kTileRows - 1));
    g.lds = g.vb >= 1 ? (size_t)planes_for(precision) * (mfma::kTileRows + g.vb - 1) * g.ldsp * 2 : 0;
@@ -378,7 +378,7 @@ Geometry pair_geometry(int KC, int NT, int th, long lds_cap = mfma::kLdsMax)
    Geometry g;
    const int lcols = mfma::pair_lcols(KC, NT);
    g.ldsp = (lcols + 7) / 8 * 8;
-   while ((g.ldsp * 2) % 128 != 16) g.ldsp += 8;
+   while ((g.ldsp * 2) % 32 != 16) g.ldsp += 8;
    const long rows_fit = lds_cap / ((long)g.ldsp * 2);
    const int S = (th + 1) / 2;
    g.vb = (int)std::min<long>(S, (rows_fit - (mfma::kTileRows - 2)) / 2);
