@@ -230,10 +230,13 @@ def configs_bench(ocip, steps):
                     f"(ocipProximityBackwardBatch)",
         "ms_per_step": round(ms, 4),
         "gradient_gpix_tpix_per_s": round(2 * 2 * n * W * H * tw * th / (ms * 1e-3) / 1e9, 1),
-        "roofline": {"bound": "valu", "achieved": round(gflop / ms, 2),
+        "roofline": {"bound": "mfma", "achieved": round(gflop / ms, 2),
                      "peak": PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
                      "frac": round(gflop / ms / PEAK_TFLOPS["f32"], 4),
-                     "note": "whole step (every kernel of the batched backward call)"}}
+                     "note": "whole step (every kernel of the batched backward call): "
+                             "F32-equivalent algorithmic FLOP, computed as split-BF16 MFMA "
+                             "correlations and contraction with FP64 window sums, against "
+                             "the FP32 peak"}}
     for i in Ss + G1 + G2 + dSs + [T, dT]:
         i.release()
     return out
