@@ -222,6 +222,21 @@ __device__ __forceinline__ f32x16 mma(u16x8 a, u16x8 b, f32x16 c)
 // LDS reads would also wait for these prefetches.  Reading through global-address-space
 // pointers gives global loads, counted by vmcnt only.
 typedef const u16x8 __attribute__((address_space(1))) *gfrag_ptr;
+// The same for the output maps and the template sums, whose pointers the batch kernels also
+// read from a table: stores through a flat pointer count against the LDS wait counter too, so
+// an epilogue's LDS reads would wait for every earlier store of the thread.
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) float gmem_f32;
+typedef __attribute__((address_space(1))) f32x4_t gmem_f32x4;
+typedef __attribute__((address_space(1))) char gmem_char;
+__device__ __forceinline__ gmem_f32 *out_row(const TileArgs &a, int y)
+{
+   return reinterpret_cast<gmem_f32 *>((gmem_char *)a.dst + (size_t)y * a.dstep);
+}
+__device__ __forceinline__ double tmpl_sqsum(const TileArgs &a)
+{
+   return *(const __attribute__((address_space(1))) double *)a.tsums;
+}
 
 template <int NPL, int KC>
 __device__ __forceinline__ void load_frags(const u16x8 *__restrict__ f0, const u16x8 *__restrict__ f1,
@@ -396,7 +411,7 @@ __device__ __forceinline__ void mfma_tile(const TileArgs &a, int bx, int by, uns
    const int n = lane & 31, mb = 4 * (lane >> 5);
    const int ybase = y0 + wave * 32 + mb;
    double tt = 0.0;
-   if (MODE == kModeNCC || MODE == kModeSSDN) tt = a.tsums[0];
+   if (MODE == kModeNCC || MODE == kModeSSDN) tt = tmpl_sqsum(a);
 #pragma unroll
    for (int j = 0; j < NT; ++j) {
       const int x = x0 + 32 * j + n;
@@ -426,7 +441,7 @@ __device__ __forceinline__ void mfma_tile(const TileArgs &a, int bx, int by, uns
       for (int i = 0; i < 16; ++i) {
          const int y = ybase + 8 * (i / 4) + (i % 4);
          if (y < H)
-            reinterpret_cast<float *>(reinterpret_cast<char *>(a.dst) + (size_t)y * a.dstep)[x] = val[i];
+            out_row(a, y)[x] = val[i];
       }
    }
 }
@@ -807,7 +822,7 @@ __device__ __forceinline__ void pair_tile(const TileArgs &a, int bx, int by, uns
    // right edge; otherwise consecutive threads take consecutive columns
    static_assert(TP % 4 == 0 && C % 4 == 0, "16-byte groups of the LDS tile");
    if constexpr (NORM) {
-      const double tt = a.tsums[0];
+      const double tt = tmpl_sqsum(a);
       const bool v4 = a.dstep % 16 == 0 && reinterpret_cast<uintptr_t>(a.dst) % 16 == 0;
       auto finish = [&](int i0, const Corners &q) {
 #pragma unroll
@@ -825,9 +840,9 @@ __device__ __forceinline__ void pair_tile(const TileArgs &a, int bx, int by, uns
                const double num = MODE == kModeNCC ? (double)cc[e] : ss - 2.0 * (double)cc[e] + tt;
                v[e] = (float)num * inv_sqrt(ss * tt);
             }
-            float *o = reinterpret_cast<float *>(reinterpret_cast<char *>(a.dst) + (size_t)y * a.dstep) + x;
+            gmem_f32 *o = out_row(a, y) + x;
             if (v4 && x + 4 <= a.W) {
-               *reinterpret_cast<float4 *>(o) = make_float4(v[0], v[1], v[2], v[3]);
+               *reinterpret_cast<gmem_f32x4 *>(o) = f32x4_t{v[0], v[1], v[2], v[3]};
             } else {
 #pragma unroll
                for (int e = 0; e < 4; ++e)
@@ -853,10 +868,10 @@ __device__ __forceinline__ void pair_tile(const TileArgs &a, int bx, int by, uns
          const int r = k / (C / 4), c = 4 * (k - r * (C / 4));
          const int y = Y0 + r, x = x0 + c;
          if (y >= a.H || x >= a.W) continue;
-         float *o = reinterpret_cast<float *>(reinterpret_cast<char *>(a.dst) + (size_t)y * a.dstep) + x;
+         gmem_f32 *o = out_row(a, y) + x;
          const float4 q = *reinterpret_cast<const float4 *>(&tile[(r + 1) * TP + c]);
          if (x + 4 <= a.W) {
-            *reinterpret_cast<float4 *>(o) = q;
+            *reinterpret_cast<gmem_f32x4 *>(o) = f32x4_t{q.x, q.y, q.z, q.w};
          } else {
             o[0] = q.x;
             if (x + 1 < a.W) o[1] = q.y;
@@ -868,8 +883,7 @@ __device__ __forceinline__ void pair_tile(const TileArgs &a, int bx, int by, uns
          const int r = k / C, c = k - r * C;
          const int y = Y0 + r, x = x0 + c;
          if (y >= a.H || x >= a.W) continue;
-         reinterpret_cast<float *>(reinterpret_cast<char *>(a.dst) + (size_t)y * a.dstep)[x] =
-             tile[(r + 1) * TP + c];
+         out_row(a, y)[x] = tile[(r + 1) * TP + c];
       }
    }
 }
