@@ -324,6 +324,27 @@ def test_int_mfma_equals_valu_path(ocip, monkeypatch, dtype, shape, tshape):
     monkeypatch.delenv("OCIP_INT_MFMA", raising=False)
 
 
+@pytest.mark.parametrize("dtype", [np.uint8, np.int8])
+def test_int_mfma_pitched_source(ocip, monkeypatch, dtype):
+    """A pitched 8-bit source whose width is not a multiple of 4 (4-byte aligned rows, so the
+    window sums stage whole dwords except where one reaches past the image's right side):
+    the MFMA path equals the VALU path bit for bit."""
+    rng = np.random.default_rng(131)
+    info = np.iinfo(dtype)
+    big = rng.integers(info.min, info.max, size=(90, 136), endpoint=True).astype(dtype)
+    src = big[:, :130]
+    assert src.strides[0] == 136
+    tpl = rng.integers(info.min, info.max, size=(17, 23), endpoint=True).astype(dtype)
+    for op in ["SqrDistance", "CrossCorr_Norm"]:
+        say("int mfma pitched", np.dtype(dtype).name, op)
+        monkeypatch.delenv("OCIP_INT_MFMA", raising=False)
+        got = run_op(ocip, op, src, tpl)
+        monkeypatch.setenv("OCIP_INT_MFMA", "0")
+        want = run_op(ocip, op, src, tpl)
+        np.testing.assert_array_equal(got, want, err_msg=op)
+    monkeypatch.delenv("OCIP_INT_MFMA", raising=False)
+
+
 @pytest.mark.parametrize("dtype,value", [(np.uint8, 255), (np.int8, -128), (np.int8, 127)])
 def test_int_mfma_largest_products(ocip, dtype, value):
     """Constant images and templates of the largest magnitude: every product is max|s t|, so
