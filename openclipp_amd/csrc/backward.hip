@@ -41,23 +41,72 @@ __device__ __forceinline__ double block_sum_d(double v, double *red)
    return t;
 }
 
-// Window sums of a map (or of its squares) in FP64: out[y * W + x] = sum over v < th, u < tw
-// of f(m(x + u - ax, y + v - ay)), f = identity or square, zero outside the image.  One
-// workgroup per 32 x 64 outputs: the map region (31 + th rows by 63 + tw columns) is staged in
-// LDS as F32; a vertical pass (thread = column) keeps two FP64 running sums down the rows from
-// the region's top, `lead` taking each row as it enters the window and `lag` each row as it
-// leaves, so the column sum V = lead - lag is a difference of two prefixes of one sequence
-// (exactly 0 for a column whose window rows are all zero); V goes to LDS in FP64; a horizontal
-// pass (thread = output row and 8 columns) does the same along the row.  Every value of f is
-// exact in FP64 (a float or the square of a float), and the prefixes start at the tile, so the
-// relative error stays near 2^-53 times (tile energy / window energy).
-constexpr int kBxRows = 32, kBxCols = 64;
-
 // Per-image arguments of the batched kernels (blockIdx.y = image of a chunk): a pointer and a
 // row pitch per image, passed by value among the kernel arguments.
 struct ZPtrs {
    const char *p[kSplitImages];
    int step[kSplitImages];
+};
+
+// Per-pixel maps of the gradient of L = sum G1 NCC + sum G2 SSD (has1 / has2: that term
+// present), with D = sqrt(SS TT) and N = C / D (the first terms 0 where D = 0):
+//   M  = G1 / D - 2 G2          (dT: its correlation with the source; dS: its correlation
+//                                with the flipped template)
+//   Eb = 2 G2 - G1 N / SS       (dS: s times its flipped window sums)
+// each formed in FP64 and rounded once; a1 and a2 gather G1 N and G2 for the block partials.
+__device__ __forceinline__ void grad_pixel(bool has1, bool has2, double c, double ss, double tt,
+                                           double g1, double g2, float &m_out, float &e_out,
+                                           double &a1, double &a2)
+{
+   double m = 0.0, e = 0.0;
+   if (has1) {
+      const double d = sqrt(ss * tt);
+      if (d > 0) {
+         const double nn = c / d;
+         m = g1 / d;
+         e = -(g1 * nn / ss);
+         a1 += g1 * nn;
+      }
+   }
+   if (has2) {
+      m -= 2.0 * g2;
+      e += 2.0 * g2;
+      a2 += g2;
+   }
+   m_out = (float)m;
+   e_out = (float)e;
+}
+
+// Window sums of a map (or of its squares) in FP64, sum over v < th, u < tw of
+// f(m(x + u - ax, y + v - ay)), f = identity or square, zero outside the image, used at once
+// by the epilogue EPI:
+//   kBoxMaps   (the windows' sums of s^2 of the sources, anchor (tw/2, th/2)): the gradient
+//              maps M and Eb of every pixel (grad_pixel) and the workgroup's partial sums;
+//   kBoxSource (the flipped window sums of Eb, anchor (tw-1-tw/2, th-1-th/2)): the source
+//              gradient dS = F + s * box, in FP64, rounded once (a null dS: nothing written).
+// One workgroup per 32 x 64 outputs of image blockIdx.y: the map region (31 + th rows by
+// 63 + tw columns) is staged in LDS as F32; a vertical pass (thread = column) keeps two FP64
+// running sums down the rows from the region's top, `lead` taking each row as it enters the
+// window and `lag` each row as it leaves, so the column sum V = lead - lag is a difference of
+// two prefixes of one sequence (exactly 0 for a column whose window rows are all zero); V goes
+// to LDS in FP64; a horizontal pass (thread = output row and 8 columns) does the same along
+// the row.  Every value of f is exact in FP64 (a float or the square of a float), and the
+// prefixes start at the tile, so the relative error stays near 2^-53 times (tile energy /
+// window energy).
+constexpr int kBxRows = 32, kBxCols = 64;
+enum { kBoxMaps = 1, kBoxSource = 2 };
+
+// Epilogue arguments of box_f64: image z's maps C (= F), M, Eb at z * zstride elements from
+// their bases (W x H, rows W apart); the gradient tables (kBoxMaps), the sources and the dS
+// outputs (kBoxSource); partial sums of image z at sncc / sssd + z * gridDim.x.
+struct BoxEpi {
+   const float *C;
+   float *M, *Eb;
+   size_t zstride;
+   ZPtrs g1, g2, src, ds;
+   int has1, has2;
+   const double *tsums;
+   double *sncc, *sssd;
 };
 
 size_t box_f64_lds(int tw, int th)
@@ -66,15 +115,16 @@ size_t box_f64_lds(int tw, int th)
    return (size_t)kBxRows * ncols * sizeof(double) + nrows * ncols * sizeof(float);
 }
 
+template <int EPI>
 __global__ __launch_bounds__(kThreads) void box_f64(ZPtrs maps, int W, int H, int tw, int th,
-                                                     int ax, int ay, int sqr, double *outs,
-                                                     size_t ostride, int tiles_x)
+                                                     int ax, int ay, int sqr, BoxEpi ep,
+                                                     int tiles_x)
 {
    extern __shared__ double bx_lds[];
-   // image blockIdx.y: its map, and its output at outs + blockIdx.y * ostride
-   const char *m = maps.p[blockIdx.y];
-   const int mstep = maps.step[blockIdx.y];
-   double *out = outs + blockIdx.y * ostride;
+   __shared__ double red[4];
+   const int z = blockIdx.y;
+   const char *m = maps.p[z];
+   const int mstep = maps.step[z];
    const int ncols = kBxCols - 1 + tw, nrows = kBxRows - 1 + th;
    double *vs = bx_lds;                                            // kBxRows x ncols
    float *ms = reinterpret_cast<float *>(vs + kBxRows * ncols);   // nrows x ncols
@@ -117,48 +167,80 @@ __global__ __launch_bounds__(kThreads) void box_f64(ZPtrs maps, int W, int H, in
    __syncthreads();
    const int r = threadIdx.x >> 3, q = (threadIdx.x & 7) * 8;
    const int y = y0 + r;
-   if (y >= H) return;
-   const double *vrow = vs + r * ncols + q;
-   double lead = 0.0, lag = 0.0;
-   for (int k = 0; k < tw - 1; ++k) lead += vrow[k];
-   double *orow = out + (size_t)y * W;
+   const size_t zo = (size_t)z * ep.zstride;
+   double a1 = 0.0, a2 = 0.0;
+   if (y < H) {
+      const double *vrow = vs + r * ncols + q;
+      double lead = 0.0, lag = 0.0;
+      for (int k = 0; k < tw - 1; ++k) lead += vrow[k];
+      const size_t ro = zo + (size_t)y * W;
+      if constexpr (EPI == kBoxMaps) {
+         const double tt = ep.tsums[0];
+         const float *g1 = ep.has1 ? reinterpret_cast<const float *>(ep.g1.p[z] + (size_t)y * ep.g1.step[z]) : nullptr;
+         const float *g2 = ep.has2 ? reinterpret_cast<const float *>(ep.g2.p[z] + (size_t)y * ep.g2.step[z]) : nullptr;
 #pragma unroll
-   for (int j = 0; j < 8; ++j) {
-      lead += vrow[j + tw - 1];
-      if (x0 + q + j < W) orow[x0 + q + j] = lead - lag;
-      lag += vrow[j];
+         for (int j = 0; j < 8; ++j) {
+            lead += vrow[j + tw - 1];
+            const int x = x0 + q + j;
+            if (x < W)
+               grad_pixel(ep.has1, ep.has2, ep.C[ro + x], lead - lag, tt, g1 ? (double)g1[x] : 0.0,
+                          g2 ? (double)g2[x] : 0.0, ep.M[ro + x], ep.Eb[ro + x], a1, a2);
+            lag += vrow[j];
+         }
+      } else {
+         char *dsrow = const_cast<char *>(ep.ds.p[z]);
+         if (dsrow) {
+            float *ds = reinterpret_cast<float *>(dsrow + (size_t)y * ep.ds.step[z]);
+            const float *srow = reinterpret_cast<const float *>(ep.src.p[z] + (size_t)y * ep.src.step[z]);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+               lead += vrow[j + tw - 1];
+               const int x = x0 + q + j;
+               if (x < W)
+                  ds[x] = (float)((double)ep.C[ro + x] + (double)srow[x] * (lead - lag));
+               lag += vrow[j];
+            }
+         }
+      }
+   }
+   if constexpr (EPI == kBoxMaps) {
+      // the workgroup's partial sums (every thread takes part in the reductions)
+      if (ep.has1) {
+         a1 = block_sum_d(a1, red);
+         if (threadIdx.x == 0) ep.sncc[(size_t)z * gridDim.x + blockIdx.x] = a1;
+      }
+      if (ep.has2) {
+         a2 = block_sum_d(a2, red);
+         if (threadIdx.x == 0) ep.sssd[(size_t)z * gridDim.x + blockIdx.x] = a2;
+      }
    }
 }
 
-// Window sums of the maps of n images (maps.p[z], z < n) into out + z * ostride
+int box_tiles(int W, int H) { return ((W + kBxCols - 1) / kBxCols) * ((H + kBxRows - 1) / kBxRows); }
+
+// box_f64<EPI> over n images (maps.p[z], z < n)
+template <int EPI>
 void launch_box_f64(hipStream_t s, const ZPtrs &maps, int n, int W, int H, int tw, int th,
-                    int ax, int ay, bool sqr, double *out, size_t ostride)
+                    int ax, int ay, bool sqr, const BoxEpi &ep)
 {
    const int tiles_x = (W + kBxCols - 1) / kBxCols;
-   const int tiles = tiles_x * ((H + kBxRows - 1) / kBxRows);
    const size_t lds = box_f64_lds(tw, th);
    if (lds > 64 * 1024)
-      hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(box_f64),
+      hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(box_f64<EPI>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                 "lds attribute");
-   box_f64<<<dim3(tiles, n), kThreads, lds, s>>>(maps, W, H, tw, th, ax, ay, sqr ? 1 : 0, out,
-                                                ostride, tiles_x);
+   box_f64<EPI><<<dim3(box_tiles(W, H), n), kThreads, lds, s>>>(maps, W, H, tw, th, ax, ay,
+                                                               sqr ? 1 : 0, ep, tiles_x);
    launch_check("box_f64");
 }
 
-// Per-pixel maps of the gradient of L = sum G1 NCC + sum G2 SSD (G1 or G2 null: that term
-// absent), with D = sqrt(SS TT) and N = C / D (the first terms 0 where D = 0):
-//   M  = G1 / D - 2 G2          (dT: its correlation with the source; dS: its correlation
-//                                with the flipped template)
-//   Eb = 2 G2 - G1 N / SS       (dS: s times its flipped window sums)
-// each formed in FP64 and rounded once, and per-block partial sums of G1 N (sncc) and of G2
-// (sssd).  SS comes from the dense window-sum map `box` (box_f64), or when that is null from
-// the exclusive FP64 integral `integ`.  Image z = blockIdx.y: C, box, M and Eb at z * zstride
-// elements from their bases (W x H, rows W apart), G1 / G2 from the tables (has1 / has2: the
-// term present), partial sums at sncc / sssd + z * gridDim.x.
+// The gradient maps (grad_pixel) of every pixel with SS from the exclusive FP64 integral
+// `integ` (templates whose box tile does not fit LDS), or without SS (has1 = 0), and
+// per-block partial sums.  Image z = blockIdx.y: C, M and Eb at z * zstride elements from their
+// bases (W x H, rows W apart), G1 / G2 from the tables, partial sums at sncc / sssd +
+// z * gridDim.x.
 __global__ __launch_bounds__(kThreads) void grad_maps(const float *Cs, ZPtrs g1s, ZPtrs g2s,
                                                        int has1, int has2,
-                                                       const double *__restrict__ boxes,
                                                        const double *__restrict__ integ,
                                                        const double *__restrict__ tsums, int W,
                                                        int H, int tw, int th, float *Ms,
@@ -168,40 +250,20 @@ __global__ __launch_bounds__(kThreads) void grad_maps(const float *Cs, ZPtrs g1s
    __shared__ double red[4];
    const size_t zo = blockIdx.y * zstride;
    const float *C = Cs + zo;
-   const double *box = boxes ? boxes + zo : nullptr;
    float *M = Ms + zo, *Eb = Ebs + zo;
    const char *G1 = has1 ? g1s.p[blockIdx.y] : nullptr, *G2 = has2 ? g2s.p[blockIdx.y] : nullptr;
    const int g1step = g1s.step[blockIdx.y], g2step = g2s.step[blockIdx.y];
    double *sncc = snccs + (size_t)blockIdx.y * gridDim.x, *sssd = sssds + (size_t)blockIdx.y * gridDim.x;
-   const int cstep = W * (int)sizeof(float);
    const double tt = tsums[0];
    const int cx = tw / 2, cy = th / 2;
    double a1 = 0, a2 = 0;
    for (long long k = (long long)blockIdx.x * kThreads + threadIdx.x; k < (long long)W * H;
         k += (long long)gridDim.x * kThreads) {
       const int y = (int)(k / W), x = (int)(k - (long long)y * W);
-      double m = 0.0, e = 0.0;
-      if (G1) {
-         const double ss = box ? box[k] : rect_sum(integ, W, H, x - cx, x - cx + tw, y - cy, y - cy + th);
-         const double d = sqrt(ss * tt);
-         const double g = reinterpret_cast<const float *>(G1 + (size_t)y * g1step)[x];
-         const double c = reinterpret_cast<const float *>(reinterpret_cast<const char *>(C) +
-                                                          (size_t)y * cstep)[x];
-         if (d > 0) {
-            const double nn = c / d;
-            m = g / d;
-            e = -(g * nn / ss);
-            a1 += g * nn;
-         }
-      }
-      if (G2) {
-         const double g = reinterpret_cast<const float *>(G2 + (size_t)y * g2step)[x];
-         m -= 2.0 * g;
-         e += 2.0 * g;
-         a2 += g;
-      }
-      M[k] = (float)m;
-      Eb[k] = (float)e;
+      const double ss = G1 ? rect_sum(integ, W, H, x - cx, x - cx + tw, y - cy, y - cy + th) : 0.0;
+      const double g1 = G1 ? reinterpret_cast<const float *>(G1 + (size_t)y * g1step)[x] : 0.0;
+      const double g2 = G2 ? reinterpret_cast<const float *>(G2 + (size_t)y * g2step)[x] : 0.0;
+      grad_pixel(G1 != nullptr, G2 != nullptr, C[k], ss, tt, g1, g2, M[k], Eb[k], a1, a2);
    }
    if (G1) {
       a1 = block_sum_d(a1, red);
@@ -533,12 +595,11 @@ __global__ void flip_template(const float *tstage, int tpitch, int tw, int th, i
    }
 }
 
-// dS(j, i) = F(j, i) + s(j, i) * (flipped window sum of a map: the dense window-sum map `bm`
-// (box_f64), or when that is null from the map's integral I), in FP64, rounded once.  Image
-// z = blockIdx.y: F and bm at z * zstride elements from their bases, source and dS from the
-// tables (a null dS: nothing to write).
+// dS(j, i) = F(j, i) + s(j, i) * (flipped window sum of Eb from its exclusive FP64 integral
+// I), in FP64, rounded once (templates whose box tile does not fit LDS; box_f64<kBoxSource>
+// otherwise).  Image z = blockIdx.y: F at z * zstride elements from its base, source and dS
+// from the tables (a null dS: nothing to write).
 __global__ __launch_bounds__(kThreads) void grad_source(const float *Fs, ZPtrs srcs,
-                                                         const double *__restrict__ bms,
                                                          const double *__restrict__ I,
                                                          size_t zstride, int W, int H, int tw,
                                                          int th, ZPtrs dss)
@@ -547,14 +608,13 @@ __global__ __launch_bounds__(kThreads) void grad_source(const float *Fs, ZPtrs s
    if (!dS) return;
    const int dsstep = dss.step[blockIdx.y];
    const float *F = Fs + blockIdx.y * zstride;
-   const double *bm = bms ? bms + blockIdx.y * zstride : nullptr;
    const float *S = reinterpret_cast<const float *>(srcs.p[blockIdx.y]);
    const int sstep = srcs.step[blockIdx.y];
    const int cx = tw / 2, cy = th / 2;
    for (long long k = (long long)blockIdx.x * kThreads + threadIdx.x; k < (long long)W * H;
         k += (long long)gridDim.x * kThreads) {
       const int i = (int)(k / W), j = (int)(k - (long long)i * W);
-      const double box = bm ? bm[k] : rect_sum(I, W, H, j + cx - tw + 1, j + cx + 1, i + cy - th + 1, i + cy + 1);
+      const double box = rect_sum(I, W, H, j + cx - tw + 1, j + cx + 1, i + cy - th + 1, i + cy + 1);
       const double s = reinterpret_cast<const float *>(reinterpret_cast<const char *>(S) +
                                                        (size_t)i * sstep)[j];
       reinterpret_cast<float *>(dS + (size_t)i * dsstep)[j] = (float)((double)F[k] + s * box);
@@ -604,12 +664,13 @@ BwLayout bw_layout(int n, int W, int H, int tw, int th)
    L.F = o;       o = align_up(o + maps);
    L.M = o;       o = align_up(o + maps);
    L.E = o;       o = align_up(o + maps);
-   // the chunk's dense W x H FP64 window-sum maps, or one exclusive integral
-   L.I = o;       o = align_up(o + std::max(excl_bytes(W, H), (size_t)L.zc * px * sizeof(double)));
+   L.I = o;       o = align_up(o + excl_bytes(W, H));   // templates too large for box_f64
    L.partial = o; o = align_up(o + (size_t)n * L.nb * tw * th * sizeof(float));
    L.red = o;     o = align_up(o + (size_t)((n * L.nb + kRedBlocks - 1) / kRedBlocks) * tw * th *
                                    sizeof(double));
-   L.scal = o;    o = align_up(o + (size_t)2 * n * 1024 * sizeof(double));
+   // per image, one partial sum of G1 N and of G2 per block of grad_maps (at most 1024) or
+   // per tile of box_f64
+   L.scal = o;    o = align_up(o + (size_t)2 * n * std::max(1024, box_tiles(W, H)) * sizeof(double));
    L.tflip = o;   o = align_up(o + (size_t)L.twf * L.thf * sizeof(float));
    L.fragT = o;   o = align_up(o + crosscorr_split_frag_bytes(tw, th));
    L.fragF = o;   o = align_up(o + crosscorr_split_frag_bytes(L.twf, L.thf));
@@ -646,13 +707,15 @@ void launch_backward_batch(hipStream_t s, int n, const View *S, const View *G1, 
    double *I = reinterpret_cast<double *>(w + L.I);
    float *partial = reinterpret_cast<float *>(w + L.partial);
    double *sncc = reinterpret_cast<double *>(w + L.scal);
-   double *sssd = sncc + (size_t)n * 1024;
+   double *sssd = sncc + (size_t)n * std::max(1024, box_tiles(W, H));
    float *tflip = reinterpret_cast<float *>(w + L.tflip);
    double *sums = reinterpret_cast<double *>(w + L.sums);
-   const int nscal = grid_for((long long)px);
-   const int taps = tw * th;
-   // window sums straight from the maps (box_f64) when its LDS tile fits, else from integrals
+   // window sums straight from the maps (box_f64, its epilogues forming the gradient maps
+   // and dS) when its LDS tile fits, else from integrals
    const bool use_box = box_f64_lds(tw, th) <= (size_t)(160 * 1024 - 256);
+   // partial sums per image: one per tile of box_f64<kBoxMaps>, else per block of grad_maps
+   const int nscal = use_box && G1 ? box_tiles(W, H) : grid_for((long long)px);
+   const int taps = tw * th;
    KernelTimer kt(s, "backward");
 
    bool any_ds = false;
@@ -703,22 +766,36 @@ void launch_backward_batch(hipStream_t s, int n, const View *S, const View *G1, 
          }
          launch_crosscorr_split(s, m, src, step, dst, W * 4, W, H, tw, th, w + L.fragT, sums);
       }
-      // the maps M and Eb (and partial sums) after the windows' sums of s^2
-      if (use_box) {
-         if (G1) launch_box_f64(s, zs, m, W, H, tw, th, tw / 2, th / 2, true, I, px);
+      // the maps M and Eb (and partial sums): from the windows' sums of s^2 in the epilogue
+      // of box_f64, or with those sums from integrals, or without them (no CrossCorr_Norm term)
+      if (use_box && G1) {
+         BoxEpi ep{};
+         ep.C = F0;
+         ep.M = M0;
+         ep.Eb = E0;
+         ep.zstride = px;
+         ep.g1 = zg1;
+         ep.g2 = zg2;
+         ep.has1 = 1;
+         ep.has2 = G2 ? 1 : 0;
+         ep.tsums = tsums;
+         ep.sncc = sncc + (size_t)c0 * nscal;
+         ep.sssd = sssd + (size_t)c0 * nscal;
+         launch_box_f64<kBoxMaps>(s, zs, m, W, H, tw, th, tw / 2, th / 2, true, ep);
+      } else if (!G1) {
          grad_maps<<<dim3(nscal, m), kThreads, 0, s>>>(
-             F0, zg1, zg2, G1 ? 1 : 0, G2 ? 1 : 0, G1 ? I : nullptr, nullptr, tsums, W, H, tw,
-             th, M0, E0, px, sncc + (size_t)c0 * nscal, sssd + (size_t)c0 * nscal);
+             F0, zg1, zg2, 0, 1, nullptr, tsums, W, H, tw, th, M0, E0, px,
+             sncc + (size_t)c0 * nscal, sssd + (size_t)c0 * nscal);
          launch_check("grad_maps");
       } else {
          for (int k = 0; k < m; ++k) {
             const int i = c0 + k;
-            if (G1) launch_integral(s, S[i], 0, true, I, 0, -1, aux);
+            launch_integral(s, S[i], 0, true, I, 0, -1, aux);
             grad_maps<<<dim3(nscal, 1), kThreads, 0, s>>>(
-                F0 + (size_t)k * px, G1 ? one(G1[i].ptr, G1[i].step) : ZPtrs{},
-                G2 ? one(G2[i].ptr, G2[i].step) : ZPtrs{}, G1 ? 1 : 0, G2 ? 1 : 0, nullptr,
-                G1 ? I : nullptr, tsums, W, H, tw, th, M0 + (size_t)k * px, E0 + (size_t)k * px, 0,
-                sncc + (size_t)i * nscal, sssd + (size_t)i * nscal);
+                F0 + (size_t)k * px, one(G1[i].ptr, G1[i].step),
+                G2 ? one(G2[i].ptr, G2[i].step) : ZPtrs{}, 1, G2 ? 1 : 0, I, tsums, W, H, tw,
+                th, M0 + (size_t)k * px, E0 + (size_t)k * px, 0, sncc + (size_t)i * nscal,
+                sssd + (size_t)i * nscal);
             launch_check("grad_maps");
          }
       }
@@ -768,10 +845,13 @@ void launch_backward_batch(hipStream_t s, int n, const View *S, const View *G1, 
       // dS from F, s and the flipped window sums of Eb (window anchored at tw - 1 - cx,
       // th - 1 - cy)
       if (use_box) {
-         launch_box_f64(s, ze, m, W, H, tw, th, tw - 1 - tw / 2, th - 1 - th / 2, false, I, px);
-         grad_source<<<dim3(grid_for((long long)px), m), kThreads, 0, s>>>(
-             F0, zs, I, nullptr, px, W, H, tw, th, zds);
-         launch_check("grad_source");
+         BoxEpi ep{};
+         ep.C = F0;
+         ep.zstride = px;
+         ep.src = zs;
+         ep.ds = zds;
+         launch_box_f64<kBoxSource>(s, ze, m, W, H, tw, th, tw - 1 - tw / 2, th - 1 - th / 2,
+                                    false, ep);
       } else {
          for (int k = 0; k < m; ++k) {
             const int i = c0 + k;
@@ -779,7 +859,7 @@ void launch_backward_batch(hipStream_t s, int n, const View *S, const View *G1, 
             const View ev{E0 + (size_t)k * px, W, H, W * 4, 1, SImage::F32};
             launch_integral(s, ev, 0, false, I, 0, -1, aux);
             grad_source<<<dim3(grid_for((long long)px), 1), kThreads, 0, s>>>(
-                F0 + (size_t)k * px, one(S[i].ptr, S[i].step), nullptr, I, 0, W, H, tw, th,
+                F0 + (size_t)k * px, one(S[i].ptr, S[i].step), I, 0, W, H, tw, th,
                 one(dS[i], dsstep[i]));
             launch_check("grad_source");
          }
