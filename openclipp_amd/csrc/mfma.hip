@@ -814,9 +814,10 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
 static_assert(kSplitImages == mfma::kZImages, "sources per launch");
 size_t crosscorr_split_frag_bytes(int tw, int th) { return mfma_frag_bytes(mfma::kBF16x3, tw, th); }
 
-static Geometry split_geometry(int tw, int th)
+// rows per LDS band for the wider of the two tile widths the autotuner tries (NT = 4)
+static Geometry split_geometry(int tw, int th, int NT = 4)
 {
-   const Geometry g = geometry(mfma::kBF16x3, kc_for(tw), 2, th);
+   const Geometry g = geometry(mfma::kBF16x3, kc_for(tw), NT, th);
    if (g.vb < 1) fail(CL_INVALID_VALUE, "template too large for the MFMA tile");
    return g;
 }
@@ -836,31 +837,43 @@ void launch_crosscorr_split(hipStream_t s, int n, const float *const *src, const
                             const void *frags, const double *sums)
 {
    const int KC = kc_for(tw), nub = nub_for(tw);
-   const Geometry g = split_geometry(tw, th);
+   const int vb = split_geometry(tw, th).vb;
    const u16x8 *f0 = static_cast<const u16x8 *>(frags);
    const u16x8 *f1 = f0 + (size_t)nub * th * KC * 64;
-   KernelTimer kt(s, "crosscorr_mfma");
-   for (int i0 = 0; i0 < n; i0 += mfma::kZImages) {
-      SingleLaunch L;
-      L.z.n = std::min(mfma::kZImages, n - i0);
-      // 16-byte staging loads: tw/2 a multiple of 4 and every source's rows 16-byte aligned
-      int vec = (tw / 2) % 4 == 0;
-      for (int k = 0; k < L.z.n; ++k) {
-         L.z.src[k] = src[i0 + k];
-         L.z.dst[k] = dst[i0 + k];
-         L.z.sstep[k] = sstep[i0 + k];
-         if (reinterpret_cast<uintptr_t>(src[i0 + k]) % 16 || sstep[i0 + k] % 16) vec = 0;
+   // one launch per kZImages sources, with NT = 2 or 4 column tiles per workgroup
+   auto run = [&](int NT) {
+      const Geometry g = split_geometry(tw, th, NT);
+      for (int i0 = 0; i0 < n; i0 += mfma::kZImages) {
+         SingleLaunch L;
+         L.z.n = std::min(mfma::kZImages, n - i0);
+         // 16-byte staging loads: tw/2 a multiple of 4 and every source's rows 16-byte aligned
+         int vec = (tw / 2) % 4 == 0;
+         for (int k = 0; k < L.z.n; ++k) {
+            L.z.src[k] = src[i0 + k];
+            L.z.dst[k] = dst[i0 + k];
+            L.z.sstep[k] = sstep[i0 + k];
+            if (reinterpret_cast<uintptr_t>(src[i0 + k]) % 16 || sstep[i0 + k] % 16) vec = 0;
+         }
+         L.mode = mfma::kModeCC;
+         L.a = mfma::TileArgs{src[i0], W, H, sstep[i0], tw, th, nub, f0, f1, sums, nullptr,
+                              dst[i0], dstep, vb, vec};
+         L.KC = KC;
+         L.NT = NT;
+         L.ldsp = g.ldsp;
+         L.lds = g.lds;
+         L.grid = dim3((W + 32 * NT - 1) / (32 * NT), (H + mfma::kTileRows - 1) / mfma::kTileRows);
+         mfma::launch_single<mfma::kBF16x3>(s, L);
       }
-      L.mode = mfma::kModeCC;
-      L.a = mfma::TileArgs{src[i0], W, H, sstep[i0], tw, th, nub, f0, f1, sums, nullptr, dst[i0],
-                           dstep, g.vb, vec};
-      L.KC = KC;
-      L.NT = 2;
-      L.ldsp = g.ldsp;
-      L.lds = g.lds;
-      L.grid = dim3((W + 63) / 64, (H + mfma::kTileRows - 1) / mfma::kTileRows);
-      mfma::launch_single<mfma::kBF16x3>(s, L);
-   }
+   };
+   // the autotuner's choice for this shape and number of sources (each candidate rewrites the
+   // same outputs)
+   char shape[64];
+   snprintf(shape, sizeof shape, "tw%d:th%d:n%d", tw, th, std::min(n, (int)mfma::kZImages));
+   const View v0{src[0], W, H, sstep[0], 1, SImage::F32};
+   const int NT = tuned_variant(tune_key("split", mfma::kBF16x3, mfma::kModeCC, shape, v0), s, run,
+                                {2, 4});
+   KernelTimer kt(s, "crosscorr_mfma");
+   run(NT);
 }
 
 // ======================================================================= exact integer sums
