@@ -373,16 +373,16 @@ Geometry geometry(int precision, int KC, int NT, int th_max)
 // (16 NT + 4) FP32 tile of the epilogue, whichever is larger.
 int kc2_for(int tw) { return tw <= 97 ? (tw + 15 + 15) / 16 : (mfma::kBlockW + 15 + 15) / 16; }
 
-Geometry pair_geometry(int KC, int NT, int th, long lds_cap = mfma::kLdsMax)
+Geometry pair_geometry(int KC, int NT, int th, long lds_cap = mfma::kLdsMax, int planes = 1)
 {
    Geometry g;
    const int lcols = mfma::pair_lcols(KC, NT);
    g.ldsp = (lcols + 7) / 8 * 8;
    while ((g.ldsp * 2) % 32 != 16) g.ldsp += 8;
-   const long rows_fit = lds_cap / ((long)g.ldsp * 2);
+   const long rows_fit = lds_cap / ((long)planes * g.ldsp * 2);
    const int S = (th + 1) / 2;
    g.vb = (int)std::min<long>(S, (rows_fit - (mfma::kTileRows - 2)) / 2);
-   const size_t staging = (size_t)(mfma::kTileRows - 2 + 2 * g.vb) * g.ldsp * 2;
+   const size_t staging = (size_t)planes * (mfma::kTileRows - 2 + 2 * g.vb) * g.ldsp * 2;
    const size_t epi = (size_t)(mfma::kTileRows + 1) * mfma::pair_tile_pitch(NT) * sizeof(float);
    g.lds = g.vb >= 1 ? std::max(staging, epi) : 0;
    return g;
@@ -617,11 +617,15 @@ static size_t mfma_frag_bytes(int precision, int tw, int th)
    return round256((size_t)planes_for(precision) * nub_for(tw) * th * kc_for(tw) * 64 * sizeof(u16x8));
 }
 
+static size_t pair_frag_count(int tw, int th)
+{
+   const int nub = nub_for(tw);
+   return (size_t)nub * mfma::pair_frag_steps(th, nub) * kc2_for(tw);
+}
+
 static size_t pair_frag_bytes(int precision, int tw, int th)
 {
-   if (precision == mfma::kBF16x3) return 0;
-   const int nub = nub_for(tw);
-   return round256((size_t)nub * mfma::pair_frag_steps(th, nub) * kc2_for(tw) * 64 * sizeof(u16x8));
+   return round256((size_t)planes_for(precision) * pair_frag_count(tw, th) * 64 * sizeof(u16x8));
 }
 
 bool crosscorr_mfma_needs_integral(int tw) { return den_chunks(tw) > kDenMaxChunks; }
@@ -691,7 +695,7 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
    const int KC = kc_for(tw), KC2 = kc2_for(tw), nub = nub_for(tw);
    if (geometry(precision, KC, 4, th).vb < 1)
       fail(CL_INVALID_VALUE, "template too large for the MFMA tile");
-   const bool pair_ok = precision != mfma::kBF16x3;
+   const bool pair_ok = true;   // every precision runs on the row-pair tile (split BF16: two planes)
    // the strip kernel's buffer loads and stores address source and output with 32-bit offsets
    // below kRingOob
    const bool strip_ok = strip_fits(precision, tw, th) &&
@@ -722,7 +726,7 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
             if (c == known->second) chosen = c;
    }
    // without tuning: the row-pair kernel with 96-column tiles (within noise of the fastest
-   // variant from 32x32 to 64x64 templates), or crosscorr_mfma for split BF16
+   // variant from 32x32 to 64x64 templates)
    if (!chosen && !may_tune) chosen = pair_ok ? 16 : 4;
 
    char *wk = static_cast<char *>(work);
@@ -730,6 +734,7 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
    u16x8 *f0 = reinterpret_cast<u16x8 *>(wk);
    u16x8 *f1 = precision == mfma::kBF16x3 ? f0 + nfrag * 64 : nullptr;
    u16x8 *g0 = reinterpret_cast<u16x8 *>(wk + mfma_frag_bytes(precision, tw, th));
+   u16x8 *g1 = precision == mfma::kBF16x3 ? g0 + pair_frag_count(tw, th) * 64 : nullptr;
    {
       // fragments of the chosen variant (of every candidate before the first tuning)
       KernelTimer kt(s, "template_prep");
@@ -740,10 +745,11 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
          else mfma::launch_single_prep<mfma::kBF16x3>(s, d);
       }
       if (pair_ok && (chosen == 0 || chosen > 10)) {
-         const BatchDesc d{static_cast<const char *>(T.ptr), tw, th, T.step, KC2, nub, g0, nullptr,
+         const BatchDesc d{static_cast<const char *>(T.ptr), tw, th, T.step, KC2, nub, g0, g1,
                            tsums};
          if (precision == mfma::kBF16) mfma::launch_pair_prep<mfma::kBF16>(s, d);
-         else mfma::launch_pair_prep<mfma::kF16>(s, d);
+         else if (precision == mfma::kF16) mfma::launch_pair_prep<mfma::kF16>(s, d);
+         else mfma::launch_pair_prep<mfma::kBF16x3>(s, d);
       }
    }
    // staged columns start at x0 - tw/2 + 64 ub + 8 g with x0 a multiple of 64 (crosscorr_mfma)
@@ -773,9 +779,9 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
       }
       if (variant > 10) {
          const int NT = variant - 10;
-         const Geometry g = pair_geometry(KC2, NT, th);
+         const Geometry g = pair_geometry(KC2, NT, th, mfma::kLdsMax, planes_for(precision));
          L.a = mfma::TileArgs{static_cast<const float *>(S.ptr), S.width, S.height, S.step, tw,
-                              th, nub, g0, nullptr, tsums, integ, dst, dstep, g.vb, vec};
+                              th, nub, g0, g1, tsums, integ, dst, dstep, g.vb, vec};
          L.KC = KC2;
          L.NT = NT;
          L.ldsp = g.ldsp;
@@ -783,7 +789,8 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
          L.grid = dim3((S.width + 16 * NT - 1) / (16 * NT),
                        (S.height + mfma::kPairRows - 1) / mfma::kPairRows);
          if (precision == mfma::kBF16) mfma::launch_pair<mfma::kBF16>(s, L);
-         else mfma::launch_pair<mfma::kF16>(s, L);
+         else if (precision == mfma::kF16) mfma::launch_pair<mfma::kF16>(s, L);
+         else mfma::launch_pair<mfma::kBF16x3>(s, L);
          return;
       }
       const int NT = variant;

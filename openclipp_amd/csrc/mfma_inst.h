@@ -36,7 +36,6 @@ template <int KC, int NT, int MODE> void batch(hipStream_t s, const BatchLaunch 
                                 L.tiles_x, L.tiles_y, L.counter, L.ldsp);
 }
 
-#if OCIP_MFMA_P != 3
 template <int KC, int NT, int MODE> void pair(hipStream_t s, const SingleLaunch &L)
 {
    auto k = crosscorr_mfma_pair<P, KC, NT, MODE>;
@@ -45,6 +44,18 @@ template <int KC, int NT, int MODE> void pair(hipStream_t s, const SingleLaunch 
    k<<<tiles, 256, L.lds, s>>>(L.a, L.ldsp, (int)L.grid.x, tiles);
 }
 
+template <int KC> void pair_mode(hipStream_t s, const SingleLaunch &L)
+{
+   if (L.mode == kModeInt) {
+      if constexpr (P == kBF16) pair<KC, 4, kModeInt>(s, L);
+      else fail(CL_INVALID_VALUE, "exact integer sums take BF16 operands");
+      return;
+   }
+   if (L.NT == 4) pair<KC, 4, kModeCC>(s, L); else pair<KC, 6, kModeCC>(s, L);
+}
+
+// the ring-buffer strip and the persistent batch kernel stage one operand plane
+#if OCIP_MFMA_P != 3
 template <int KC, bool VEC> void strip(hipStream_t s, const SingleLaunch &L, int tiles_y, int q_rows)
 {
    auto k = crosscorr_mfma_strip<P, KC, 4, kModeCC, VEC>;
@@ -76,16 +87,6 @@ template <int KC, int MODE> void pair_batch_nt(hipStream_t s, const BatchLaunch 
 template <int KC> void pair_batch_mode(hipStream_t s, const BatchLaunch &L)
 {
    if (L.mode == kModeNCC) pair_batch_nt<KC, kModeNCC>(s, L); else pair_batch_nt<KC, kModeSSDN>(s, L);
-}
-
-template <int KC> void pair_mode(hipStream_t s, const SingleLaunch &L)
-{
-   if (L.mode == kModeInt) {
-      if constexpr (P == kBF16) pair<KC, 4, kModeInt>(s, L);
-      else fail(CL_INVALID_VALUE, "exact integer sums take BF16 operands");
-      return;
-   }
-   if (L.NT == 4) pair<KC, 4, kModeCC>(s, L); else pair<KC, 6, kModeCC>(s, L);
 }
 #endif
 
@@ -141,7 +142,6 @@ template <> void launch_batch<P>(hipStream_t s, const BatchLaunch &L)
    launch_check("crosscorr_mfma_batch");
 }
 
-#if OCIP_MFMA_P != 3
 template <> void launch_pair<P>(hipStream_t s, const SingleLaunch &L)
 {
    switch (L.KC) {
@@ -157,6 +157,14 @@ template <> void launch_pair<P>(hipStream_t s, const SingleLaunch &L)
    launch_check("crosscorr_mfma_pair");
 }
 
+template <> void launch_pair_prep<P>(hipStream_t s, const BatchDesc &d)
+{
+   const int nfrag = d.nub * pair_frag_steps(d.th, d.nub) * d.KC;
+   pair_prep<P><<<(nfrag + 3) / 4 + 1, 256, 0, s>>>(d);   // + 1: the sum of squares
+   launch_check("pair_prep");
+}
+
+#if OCIP_MFMA_P != 3
 template <> void launch_pair_batch<P>(hipStream_t s, const BatchLaunch &L)
 {
    switch (L.KC) {
@@ -190,13 +198,6 @@ template <> void launch_strip<P>(hipStream_t s, const SingleLaunch &L, int tiles
    default: fail(CL_INVALID_VALUE, "template width");
    }
    launch_check("crosscorr_mfma_strip");
-}
-
-template <> void launch_pair_prep<P>(hipStream_t s, const BatchDesc &d)
-{
-   const int nfrag = d.nub * pair_frag_steps(d.th, d.nub) * d.KC;
-   pair_prep<P><<<(nfrag + 3) / 4 + 1, 256, 0, s>>>(d);   // + 1: the sum of squares
-   launch_check("pair_prep");
 }
 #endif
 

@@ -532,7 +532,7 @@ __host__ __device__ constexpr int pair_frag_steps(int th, int nub)
 template <int P>
 __device__ __forceinline__ void build_pair_fragment(const char *tptr, int tstep, int tw, int th,
                                                     int KC, int nub, int f, int lane, u16x8 *frag,
-                                                    int src8)
+                                                    u16x8 *frag1, int src8)
 {
    const int S = pair_frag_steps(th, nub);
    const int kc = f % KC, s = (f / KC) % S, ub = f / (KC * S);
@@ -541,7 +541,7 @@ __device__ __forceinline__ void build_pair_fragment(const char *tptr, int tstep,
    const int u0 = nub > 1 ? ub * kBlockW : 0;
    const int wb = nub > 1 ? min(kBlockW, tw - u0) : tw;
    const char *trow = tptr + (size_t)(v < th ? v : 0) * tstep;
-   u16x8 h;
+   u16x8 h, l;
 #pragma unroll
    for (int i = 0; i < 8; ++i) {
       const int u = 16 * kc + kb + i - dx;
@@ -549,8 +549,10 @@ __device__ __forceinline__ void build_pair_fragment(const char *tptr, int tstep,
       unsigned short h0, h1;
       encode<P>(t, h0, h1);
       h[i] = h0;
+      l[i] = h1;
    }
    frag[(size_t)f * 64 + lane] = h;
+   if (P == kBF16x3) frag1[(size_t)f * 64 + lane] = l;   // split BF16: the lo plane
 }
 
 // Fragments of the row-pair kernel, one per wavefront, the FP64 sum of squares (same order
@@ -567,7 +569,7 @@ __global__ __launch_bounds__(256) void pair_prep(BatchDesc d)
    const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
    if (f < nfrag)
       build_pair_fragment<P>(d.tptr, d.tstep, d.tw, d.th, d.KC, d.nub, f, threadIdx.x & 63,
-                             d.frag0, d.src8);
+                             d.frag0, d.frag1, d.src8);
 }
 
 // Batch: for template blockIdx.y, workgroups 0 .. gridDim.x - 2 build its fragments (in the
@@ -586,26 +588,45 @@ __global__ __launch_bounds__(256) void batch_prep(const BatchDesc *__restrict__ 
    for (int k = blockIdx.x * 256 + threadIdx.x; k < nfrag * 64; k += (gridDim.x - 1) * 256) {
       if constexpr (PAIR)
          build_pair_fragment<P>(d.tptr, d.tstep, d.tw, d.th, d.KC, d.nub, k >> 6, k & 63,
-                                d.frag0, d.src8);
+                                d.frag0, d.frag1, d.src8);
       else
          build_fragment<P>(d.tptr, d.tstep, d.tw, d.th, d.KC, d.nub, k >> 6, k & 63, d.frag0,
                            d.frag1);
    }
 }
 
-// One step: NT + KC - 1 A fragments from the LDS row `ab`, KC B fragments.
+// One step: NT + KC - 1 A fragments from the LDS row `ab`, KC B fragments.  Split BF16: the
+// lo planes of A (at `plane` past the hi plane) and B too, three products per chunk (hi hi,
+// hi lo, lo hi, in that order for each accumulator), consecutive MFMAs on different
+// accumulators.
 template <int P, int KC, int NT>
 __device__ __forceinline__ void pair_mfma(f32x16 (&acc)[NT], const unsigned short *ab,
-                                          const u16x8 (&b)[KC])
+                                          size_t plane, const u16x8 (&b)[KC],
+                                          const u16x8 (&b1)[KC])
 {
    constexpr int NA = NT + KC - 1;
    u16x8 a0[NA];
 #pragma unroll
    for (int c = 0; c < NA; ++c) a0[c] = *reinterpret_cast<const u16x8 *>(ab + 16 * c);
+   if constexpr (P != kBF16x3) {
 #pragma unroll
-   for (int kc = 0; kc < KC; ++kc)
+      for (int kc = 0; kc < KC; ++kc)
 #pragma unroll
-      for (int j = 0; j < NT; ++j) acc[j] = mma<P>(a0[j + kc], b[kc], acc[j]);
+         for (int j = 0; j < NT; ++j) acc[j] = mma<P>(a0[j + kc], b[kc], acc[j]);
+   } else {
+      u16x8 a1[NA];
+#pragma unroll
+      for (int c = 0; c < NA; ++c) a1[c] = *reinterpret_cast<const u16x8 *>(ab + plane + 16 * c);
+#pragma unroll
+      for (int kc = 0; kc < KC; ++kc) {
+#pragma unroll
+         for (int j = 0; j < NT; ++j) acc[j] = mma<P>(a0[j + kc], b[kc], acc[j]);
+#pragma unroll
+         for (int j = 0; j < NT; ++j) acc[j] = mma<P>(a0[j + kc], b1[kc], acc[j]);
+#pragma unroll
+         for (int j = 0; j < NT; ++j) acc[j] = mma<P>(a1[j + kc], b[kc], acc[j]);
+      }
+   }
 }
 
 // Output tile (bx, by): 127 rows x 16 NT columns; a.vb = steps (template row pairs) per LDS
@@ -626,11 +647,14 @@ template <int P, int KC, int NT, int MODE>
 __device__ __forceinline__ void pair_tile(const TileArgs &a, int bx, int by, unsigned short *lds,
                                           int ldsp)
 {
-   static_assert(P != kBF16x3, "one operand plane");
    static_assert(MODE == kModeCC || MODE == kModeNCC || MODE == kModeSSDN ||
                      (MODE == kModeInt && P == kBF16 && NT == 4),
                  "float maps, or exact integer sums (BF16, 4 n-tiles)");
    constexpr bool INT = MODE == kModeInt;
+   // operand planes: two for split BF16 (hi and lo), the lo plane of the staged rows at
+   // `plane` 16-bit elements past the hi plane (the rows a band of a.vb steps stages)
+   constexpr int NPL = P == kBF16x3 ? 2 : 1;
+   const size_t plane = NPL == 2 ? (size_t)(kTileRows - 2 + 2 * a.vb) * ldsp : 0;
    constexpr int lcols = pair_lcols(KC, NT);
    constexpr int C = 16 * NT;
    constexpr int TP = pair_tile_pitch(NT);
@@ -676,7 +700,7 @@ __device__ __forceinline__ void pair_tile(const TileArgs &a, int bx, int by, uns
          const int ns = min(a.vb, S - s0);
          __syncthreads();
          // LDS row r = source row Y0 - cy + 2 s0 + r; wave w reads rows 32 w + m + 2 (s - s0)
-         stage_rows<P, lcols, INT>(a, lds, ldsp, 0, Y0 - cy + 2 * s0, x0 - cx + ub * kBlockW,
+         stage_rows<P, lcols, INT>(a, lds, ldsp, plane, Y0 - cy + 2 * s0, x0 - cx + ub * kBlockW,
                                    kTileRows - 2 + 2 * ns);
          __syncthreads();
 
@@ -685,23 +709,23 @@ __device__ __forceinline__ void pair_tile(const TileArgs &a, int bx, int by, uns
          if (Y0 - 1 + 32 * wave >= a.H) continue;
          // ping-pong: the B fragments of step s+1 load while step s multiplies
          const size_t fb0 = ((size_t)(ub * pair_frag_steps(a.th, a.nub) + s0) * KC) * 64 + lane;
-         u16x8 pa[KC], pb[KC], dummy[KC];
-         load_frags<1, KC>(a.frag0, nullptr, fb0, pa, dummy);
+         u16x8 pa[KC], pb[KC], pa1[KC], pb1[KC];   // (pa1, pb1: the lo planes, split BF16)
+         load_frags<NPL, KC>(a.frag0, a.frag1, fb0, pa, pa1);
          const unsigned short *arow = lds + (size_t)(wave * 32 + (lane & 31)) * ldsp + 8 * (lane >> 5);
          int q = 0;
          for (; q + 1 < ns; q += 2) {
-            load_frags<1, KC>(a.frag0, nullptr, fb0 + (size_t)(q + 1) * KC * 64, pb, dummy);
+            load_frags<NPL, KC>(a.frag0, a.frag1, fb0 + (size_t)(q + 1) * KC * 64, pb, pb1);
             __builtin_amdgcn_sched_barrier(0);
-            pair_mfma<P, KC, NT>(acc, arow + (size_t)(2 * q) * ldsp, pa);
+            pair_mfma<P, KC, NT>(acc, arow + (size_t)(2 * q) * ldsp, plane, pa, pa1);
             stepped();
             const int qn = q + 2 < ns ? q + 2 : q + 1;
-            load_frags<1, KC>(a.frag0, nullptr, fb0 + (size_t)qn * KC * 64, pa, dummy);
+            load_frags<NPL, KC>(a.frag0, a.frag1, fb0 + (size_t)qn * KC * 64, pa, pa1);
             __builtin_amdgcn_sched_barrier(0);
-            pair_mfma<P, KC, NT>(acc, arow + (size_t)(2 * q + 2) * ldsp, pb);
+            pair_mfma<P, KC, NT>(acc, arow + (size_t)(2 * q + 2) * ldsp, plane, pb, pb1);
             stepped();
          }
          if (q < ns) {
-            pair_mfma<P, KC, NT>(acc, arow + (size_t)(2 * q) * ldsp, pa);
+            pair_mfma<P, KC, NT>(acc, arow + (size_t)(2 * q) * ldsp, plane, pa, pa1);
             stepped();
          }
       }
@@ -888,11 +912,18 @@ __device__ __forceinline__ void pair_tile(const TileArgs &a, int bx, int by, uns
    }
 }
 
+// Waves per SIMD the row-pair kernel's registers are sized for: 3 for the narrow one-plane
+// CrossCorr tile, 1 for the widest split-BF16 tiles (whose two staged planes leave LDS for one
+// workgroup per CU at these template widths, and whose three fragment sets would spill at 2).
+__host__ __device__ constexpr int pair_waves(int P, int KC, int NT, int MODE)
+{
+   return NT <= 4 && MODE == kModeCC && P != kBF16x3 ? 3 : P == kBF16x3 && NT == 6 && KC >= 7 ? 1 : 2;
+}
+
 // One 127 x 16 NT tile per workgroup, XCD-aware order (the single-template call).
 template <int P, int KC, int NT, int MODE>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(NT <= 4 && MODE == kModeCC ? 3 : 2,
-                                   NT <= 4 && MODE == kModeCC ? 3 : 2)))
+__attribute__((amdgpu_waves_per_eu(pair_waves(P, KC, NT, MODE), pair_waves(P, KC, NT, MODE))))
 void crosscorr_mfma_pair(TileArgs a, int ldsp, int tiles_x, int tiles)
 {
    extern __shared__ u16x8 lds_v[];

@@ -2,10 +2,10 @@
 variant it cannot run.
 
 A first child process tunes three calls and writes their keys to the cache file.  The file is
-then rewritten: the 128-wide BF16 template's entry names the strip kernel (valid only up to 97
-columns; before the key held the template width, a 64-wide template's tuning could be reused
-for it), the split-BF16 entry a row-pair kernel (which has no split-BF16 form), and the 64-wide
-BF16 entry the strip kernel (which it can run).  A second child runs the same calls from that
+then rewritten so that every entry names the strip kernel: the 128-wide BF16 template cannot
+run it (valid only up to 97 columns; before the key held the template width, a 64-wide
+template's tuning could be reused for it), split BF16 cannot (its ring holds one operand
+plane), and the 64-wide BF16 template can.  A second child runs the same calls from that
 file: each map must be within the MFMA bound of the FP64 oracle, and the two calls whose
 entries they cannot run must have tuned again (new entries appended).
 """
@@ -41,12 +41,7 @@ def test_cache_entries_that_a_call_cannot_run_are_tuned_again(tmp_path):
     lines = [ln.split() for ln in cache.read_text().splitlines() if ln.strip()]
     single = [k for k, _ in lines if k.startswith("single:")]
     assert len(single) == len(child.CASES)
-    bad = []
-    for key in single:
-        if ":p3:" in key:
-            bad.append(f"{key} 14")            # split BF16 has no row-pair kernel
-        else:
-            bad.append(f"{key} 24")            # strip kernel, impossible at tw = 128
+    bad = [f"{key} 24" for key in single]      # the strip kernel
     cache.write_text("\n".join(bad) + "\n")
     say("autotune child 2")
     maps = run_child(cache, tmp_path / "b.npz")

@@ -156,7 +156,7 @@ VARIANT_CASES = [((150, 260), (1, 1)), ((150, 260), (64, 64)), ((300, 200), (31,
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
-@pytest.mark.parametrize("prec", ["bf16", "f16"])
+@pytest.mark.parametrize("prec", ["bf16", "f16", "bf16x3"])
 def test_kernel_variants(ocip, monkeypatch, prec, variant):
     """Every tile variant the autotuner may pick, forced through OCIP_MFMA_VARIANT: CC within
     the operand-precision bound of the FP64 oracle, NCC within the bound over the exact
@@ -187,22 +187,24 @@ def test_kernel_variants(ocip, monkeypatch, prec, variant):
     assert abs(float(got[132, 182]) - 1.0) <= 2 * UNIT[prec]
 
 
-@pytest.mark.parametrize("prec", ["bf16", "f16"])
+@pytest.mark.parametrize("prec", ["bf16", "f16", "bf16x3"])
 def test_row_pair_variants_are_bit_identical(ocip, monkeypatch, prec):
     """The row-pair variants (14, 16: crosscorr_mfma_pair with 4 or 6 n-tiles; 24: the strip
-    kernel) add each output's products in the same order (template-row pairs in turn, chunks in
-    turn, then the two half sums), so their maps are equal bit for bit, for CrossCorr and for
-    CrossCorr_Norm (the same in-place scale)."""
+    kernel, one operand plane, so not for split BF16) add each output's products in the same
+    order (template-row pairs in turn, chunks in turn, then the two half sums; for split BF16
+    the three products of a chunk in turn), so their maps are equal bit for bit, for CrossCorr
+    and for CrossCorr_Norm (the same in-place scale)."""
     rng = np.random.default_rng(77)
     src = rng.random((300, 330), dtype=np.float32)
     tpl = rng.random((40, 52), dtype=np.float32)      # strip schedule fits: 20 steps, KC2 = 5
+    variants = (14, 16) if prec == "bf16x3" else (14, 16, 24)
     maps = {}
-    for v in (14, 16, 24):
+    for v in variants:
         monkeypatch.setenv("OCIP_MFMA_VARIANT", str(v))
         say("row-pair identity", prec, v)
         maps[v] = (run_op(ocip, "CrossCorr", src, tpl, precision=PREC[prec]),
                    run_op(ocip, "CrossCorr_Norm", src, tpl, precision=PREC[prec]))
-    for v in (16, 24):
+    for v in variants[1:]:
         np.testing.assert_array_equal(maps[v][0], maps[14][0])
         np.testing.assert_array_equal(maps[v][1], maps[14][1])
 
