@@ -894,101 +894,180 @@ static int int_max_product(int type) { return type == SImage::U8 ? 255 * 255 : 1
 
 // Window sums of s^2 of an 8-bit image, exact in int32, into the map `dst` (as int32 bits):
 // SS(x, y) = sum over the template window of (x, y) (origin (-tw/2, -th/2), zero outside the
-// image).  One workgroup per 128 x 64 outputs: the squares of the source region (127 + th
-// rows by 63 + tw columns, th and tw <= 97) are staged in LDS as 16-bit integers
-// (255^2 < 2^16; wave = rows, lane = columns, batches of 24 loads); a vertical pass (one task per 64-column chunk and half of the output rows:
-// lane = column) forms the half's first column sum over th rows and slides it down (adding
-// the row entering, subtracting the row leaving: integer arithmetic, exact in any order) into
-// an int32 LDS array; a horizontal pass (thread = output row and 32 columns) slides along the
-// rows the same way and stores 16-byte groups where the map's rows allow.
-constexpr int kBoxRows = 128, kBoxCols = 64;
+// image).  One wavefront per 64 output columns and a band of kBoxBand output rows, streaming
+// down the band's th - 1 + kBoxBand source rows.  For each row:
+//  * the squares q of its 63 + tw columns from x0 - tw/2, two adjacent columns per lane (NCH
+//    chunks of 128 columns; zero outside the image through the range of a buffer resource);
+//  * their exclusive prefix along the row, P[i] = sum of q over columns < i: a wavefront
+//    inclusive scan of the lanes' pair sums (DPP row shifts and row broadcasts), the chunk
+//    carry through readlane; integers, exact;
+//  * for output column x0 + x (lane x from here on), R(x) = P[x + tw] - P[x], both prefixes
+//    fetched from the lanes that hold them with ds_bpermute;
+//  * a per-lane ring of the last th rows' R in LDS: V += R - R(th rows up); from the th-th
+//    row on, V is the sum of the output row th - 1 rows up, stored as one 256-byte row segment.
+// Integer arithmetic throughout (every value below 2^31, checked by the host), so the sums are
+// exact in any order.
+constexpr int kBoxBand = 64;
+constexpr int kBoxOob = 0x40000000;   // a byte offset past any image this kernel takes
 
-__global__ __launch_bounds__(256) void ss_box8(const unsigned char *__restrict__ src, int W,
-                                               int H, int sstep, int tw, int th, int is_signed,
-                                               char *__restrict__ dst, int dstep, int tiles_x,
-                                               int v4)
+// Inclusive scans across the 64 lanes of N values at once (the N scans interleaved, so that
+// each DPP step's wait on the previous one is filled by the others): within rows of 16 lanes
+// (lanes without a source add 0), then rows 1 and 3 take the last lane of rows 0 and 2, rows
+// 2 and 3 the last lane of row 1.
+template <int N>
+__device__ __forceinline__ void wave_incl_scan(int (&x)[N])
 {
-   extern __shared__ int box_lds[];
-   const int ncols = kBoxCols - 1 + tw, nrows = kBoxRows - 1 + th;
-   unsigned short *sq = reinterpret_cast<unsigned short *>(box_lds);        // nrows x ncols
-   int *vs = box_lds + (nrows * ncols + 1) / 2;                              // 128 x ncols
-   const int bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
-   const int x0 = bx * kBoxCols, y0 = by * kBoxRows;
-   const int c0 = x0 - tw / 2, r0 = y0 - th / 2;
-   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-   // eight rows by up to three 64-column chunks per wave and batch: 24 loads in flight
-   for (int rbase = wave; rbase < nrows; rbase += 32) {
-      int v[8][3];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-         const int r = rbase + 4 * k, sy = r0 + r;
-         const bool rowin = r < nrows && sy >= 0 && sy < H;
-         const unsigned char *srow = src + (size_t)(rowin ? sy : 0) * sstep;
+   for (int k = 0; k < N; ++k) x[k] += __builtin_amdgcn_update_dpp(0, x[k], 0x111, 0xf, 0xf, false);
 #pragma unroll
-         for (int j = 0; j < 3; ++j) {
-            const int c = lane + 64 * j, sx = c0 + c;
-            v[k][j] = 0;
-            if (rowin && c < ncols && sx >= 0 && sx < W)
-               v[k][j] = is_signed ? (int)(signed char)srow[sx] : (int)srow[sx];
+   for (int k = 0; k < N; ++k) x[k] += __builtin_amdgcn_update_dpp(0, x[k], 0x112, 0xf, 0xf, false);
+#pragma unroll
+   for (int k = 0; k < N; ++k) x[k] += __builtin_amdgcn_update_dpp(0, x[k], 0x114, 0xf, 0xf, false);
+#pragma unroll
+   for (int k = 0; k < N; ++k) x[k] += __builtin_amdgcn_update_dpp(0, x[k], 0x118, 0xf, 0xf, false);
+#pragma unroll
+   for (int k = 0; k < N; ++k) x[k] += __builtin_amdgcn_update_dpp(0, x[k], 0x142, 0xa, 0xf, false);
+#pragma unroll
+   for (int k = 0; k < N; ++k) x[k] += __builtin_amdgcn_update_dpp(0, x[k], 0x143, 0xc, 0xf, false);
+}
+
+template <int NCH, bool SIGNED, bool BST>
+__global__ __launch_bounds__(64) void ss_box8(const unsigned char *__restrict__ src, int W,
+                                              int H, int sstep, int tw, int th,
+                                              char *__restrict__ dst, int dstep, int bands)
+{
+   extern __shared__ int ring[];   // th x 64: lane's R of the last th rows
+   constexpr int U = 8;            // rows per batch of loads (two batches in flight)
+   const int lane = threadIdx.x;
+   // consecutive blocks (one XCD's run) side by side along a band share halo columns in L2
+   const int b = xcd_tile(blockIdx.x, gridDim.x);
+   const int cblks = gridDim.x / bands;
+   const int band = b / cblks, cblk = b % cblks;
+   const int x0 = cblk * 64, y0 = band * kBoxBand;
+   const int cx = tw / 2, cy = th / 2;
+   const int n = min(kBoxBand, H - y0);
+   const int steps = n + th - 1;
+   // lane holds columns x0 - cx + 128 k + 2 lane + e (e = 0, 1) of chunk k
+   int coff[NCH][2];
+#pragma unroll
+   for (int k = 0; k < NCH; ++k)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+         const int c = x0 - cx + 128 * k + 2 * lane + e;
+         coff[k][e] = c >= 0 && c < W ? c : kBoxOob;
+      }
+   const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+       const_cast<unsigned char *>(src), (short)0, H * sstep, 0x00020000);
+   // (bytes kept as loaded until used, so that the loop carries no conversion that would wait
+   // for the loads in flight).  A row outside the image takes the row offset 2 kBoxOob, a
+   // column outside it the column offset kBoxOob: every sum with either is past the range (and
+   // below 2^32); rows past the band's last step load real rows whose sums are never used.
+   auto load = [&](int j0, unsigned char (&v)[U][NCH][2]) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+         const int sy = y0 - cy + j0 + u;
+         const unsigned roff = (unsigned)sy < (unsigned)H ? (unsigned)(sy * sstep) : 2u * kBoxOob;
+#pragma unroll
+         for (int k = 0; k < NCH; ++k)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+               const unsigned o = roff + (unsigned)coff[k][e];
+               v[u][k][e] = __builtin_amdgcn_raw_buffer_load_b8(rsrc, (int)o, 0, 0);
+            }
+      }
+   };
+   // P[x] at lane x / 2 of chunk 0 (element x % 2); P[x + tw] at lane ((x + tw) % 128) / 2
+   // of chunk (x + tw) / 128 (element (x + tw) % 2)
+   const int ia = lane, ib = lane + tw;
+   const int la = (ia >> 1) * 4, lb = ((ib & 127) >> 1) * 4;
+   const bool oa = ia & 1, ob = ib & 1, cb = ib >= 128;
+   int *const myring = ring + lane;
+   int V = 0, slot = 0;
+   // BST (maps of fewer than kBoxOob bytes): buffer stores, the column offset kBoxOob past
+   // the image's width dropping a lane's store; otherwise stores through a pointer
+   int *orow = reinterpret_cast<int *>(dst + (size_t)y0 * dstep) + x0 + lane;
+   const bool xin = x0 + lane < W;
+   const __amdgpu_buffer_rsrc_t drsrc =
+       __builtin_amdgcn_make_buffer_rsrc(dst, (short)0, BST ? H * dstep : 0, 0x00020000);
+   const int xoff = xin ? (x0 + lane) * 4 : kBoxOob;
+   int yoff = y0 * dstep;
+   unsigned char va[U][NCH][2], vb[U][NCH][2];
+   auto process = [&](int j0, const unsigned char (&v)[U][NCH][2]) {
+      // the batch's row sums first (independent rows: their scans and lane exchanges
+      // interleave, two rows at a time; rows past the band read real rows or zeros, and their
+      // sums are never used), then the ring and the running sums in order
+      int R[U];
+#pragma unroll
+      for (int u = 0; u < U; u += 2) {
+         int q0[2 * NCH], t[2 * NCH], inc[2 * NCH];
+#pragma unroll
+         for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int k = 0; k < NCH; ++k) {
+               int e0 = v[u + h][k][0], e1 = v[u + h][k][1];
+               if (SIGNED) {
+                  e0 = (int)(signed char)e0;
+                  e1 = (int)(signed char)e1;
+               }
+               q0[h * NCH + k] = e0 * e0;
+               t[h * NCH + k] = inc[h * NCH + k] = e0 * e0 + e1 * e1;
+            }
+         wave_incl_scan<2 * NCH>(inc);
+#pragma unroll
+         for (int h = 0; h < 2; ++h) {
+            int E0[NCH], E1[NCH];
+            int carry = 0;
+#pragma unroll
+            for (int k = 0; k < NCH; ++k) {
+               const int i = inc[h * NCH + k] + carry;
+               E0[k] = i - t[h * NCH + k];
+               E1[k] = E0[k] + q0[h * NCH + k];
+               carry = __builtin_amdgcn_readlane(i, 63);
+            }
+            const int a0 = __builtin_amdgcn_ds_bpermute(la, E0[0]);
+            const int a1 = __builtin_amdgcn_ds_bpermute(la, E1[0]);
+            int b0 = __builtin_amdgcn_ds_bpermute(lb, E0[0]);
+            int b1 = __builtin_amdgcn_ds_bpermute(lb, E1[0]);
+            if constexpr (NCH == 2) {
+               // P[x + tw] in chunk 1 for the lanes with x + tw >= 128
+               const int c0 = __builtin_amdgcn_ds_bpermute(lb, E0[1]);
+               const int c1 = __builtin_amdgcn_ds_bpermute(lb, E1[1]);
+               b0 = cb ? c0 : b0;
+               b1 = cb ? c1 : b1;
+            }
+            R[u + h] = (ob ? b1 : b0) - (oa ? a1 : a0);
          }
       }
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-         const int r = rbase + 4 * k;
-#pragma unroll
-         for (int j = 0; j < 3; ++j) {
-            const int c = lane + 64 * j;
-            if (r < nrows && c < ncols) sq[r * ncols + c] = (unsigned short)(v[k][j] * v[k][j]);
+      for (int u = 0; u < U; ++u) {
+         const int j = j0 + u;
+         if (j >= steps) break;
+         const int old = myring[slot * 64];
+         myring[slot * 64] = R[u];
+         V += R[u] - (j >= th ? old : 0);
+         if (++slot == th) slot = 0;
+         if (j >= th - 1) {
+            if (BST) {
+               __builtin_amdgcn_raw_buffer_store_b32(V, drsrc, (int)((unsigned)yoff + (unsigned)xoff), 0, 0);
+               yoff += dstep;
+            } else {
+               if (xin) *orow = V;
+               orow = reinterpret_cast<int *>(reinterpret_cast<char *>(orow) + dstep);
+            }
          }
       }
-   }
-   __syncthreads();
-   const int nchunks = (ncols + 63) / 64;
-   for (int task = wave; task < 2 * nchunks; task += 4) {
-      const int c = (task >> 1) * 64 + lane, rb = (task & 1) * (kBoxRows / 2);
-      if (c >= ncols) continue;
-      int v = 0;
-#pragma unroll 8
-      for (int i = 0; i < th; ++i) v += sq[(rb + i) * ncols + c];
-      vs[rb * ncols + c] = v;
-#pragma unroll 9
-      for (int r = rb + 1; r < rb + kBoxRows / 2; ++r) {
-         v += (int)sq[(r + th - 1) * ncols + c] - (int)sq[(r - 1) * ncols + c];
-         vs[r * ncols + c] = v;
-      }
-   }
-   __syncthreads();
-   const int r = threadIdx.x >> 1, q = (threadIdx.x & 1) * 32;
-   const int y = y0 + r;
-   if (y >= H) return;
-   const int *vrow = vs + r * ncols;
-   int h = 0;
-#pragma unroll 8
-   for (int k = 0; k < tw; ++k) h += vrow[q + k];
-   int out[32];
-   out[0] = h;
-#pragma unroll
-   for (int j = 1; j < 32; ++j) {
-      h += vrow[q + j + tw - 1] - vrow[q + j - 1];
-      out[j] = h;
-   }
-   int *orow = reinterpret_cast<int *>(dst + (size_t)y * dstep) + x0 + q;
-   if (v4 && x0 + q + 32 <= W) {
-#pragma unroll
-      for (int j = 0; j < 32; j += 4)
-         *reinterpret_cast<int4 *>(orow + j) = make_int4(out[j], out[j + 1], out[j + 2], out[j + 3]);
-   } else {
-#pragma unroll
-      for (int j = 0; j < 32; ++j)
-         if (x0 + q + j < W) orow[j] = out[j];
+   };
+   load(0, va);
+   for (int j0 = 0; j0 < steps; j0 += 2 * U) {
+      load(j0 + U, vb);
+      process(j0, va);
+      load(j0 + 2 * U, va);
+      process(j0 + U, vb);
    }
 }
 
-static size_t ss_box8_lds(int tw, int th)
-{
-   const size_t ncols = kBoxCols - 1 + tw, nrows = kBoxRows - 1 + th;
-   return (nrows * ncols + 1) / 2 * 4 + (size_t)kBoxRows * ncols * 4;
-}
+// chunks of 128 columns: P up to index 63 + tw (the 63 + tw staged columns and one past)
+static int ss_box8_chunks(int tw) { return 63 + tw < 128 ? 1 : 2; }
 
 bool int_mfma_ok(int op, const View &S, const View &T)
 {
@@ -997,11 +1076,12 @@ bool int_mfma_ok(int op, const View &S, const View &T)
    if ((S.type != SImage::U8 && S.type != SImage::S8) || S.channels != 1 || T.type != S.type)
       return false;
    const int tw = T.width, th = T.height;
-   // (ss_box8 stages at most 127 + 97 rows)
+   // (templates of one column block; ss_box8: byte offsets of the image below kBoxOob)
    if (tw <= 0 || th <= 0 || tw > 97 || th > 97) return false;
+   if ((long long)S.height * S.step >= kBoxOob) return false;
    const long long mp = int_max_product(S.type);
    if ((long long)tw * mp >= (1LL << 24) || (long long)tw * th * mp >= (1LL << 31)) return false;
-   return pair_geometry(kc2_for(tw), 4, th).vb >= 1 && ss_box8_lds(tw, th) <= mfma::kLdsMax;
+   return pair_geometry(kc2_for(tw), 4, th).vb >= 1;
 }
 
 size_t int_mfma_work_bytes(int tw, int th) { return pair_frag_bytes(mfma::kBF16, tw, th); }
@@ -1014,17 +1094,25 @@ void launch_int_mfma(hipStream_t s, int op, const View &S, const View &T, double
    if (op != kCC) {
       // the window sums of s^2 go to the map first; the MFMA kernel's epilogue reads them
       KernelTimer kt(s, "ss_box8");
-      const int tiles_x = (S.width + kBoxCols - 1) / kBoxCols;
-      const int tiles = tiles_x * ((S.height + kBoxRows - 1) / kBoxRows);
-      const size_t lds = ss_box8_lds(tw, th);
-      if (lds > 64 * 1024)
-         hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(ss_box8),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
-                   "lds attribute");
-      const int v4 = dstep % 16 == 0 && reinterpret_cast<uintptr_t>(dst) % 16 == 0;
-      ss_box8<<<tiles, 256, lds, s>>>(static_cast<const unsigned char *>(S.ptr), S.width,
-                                      S.height, S.step, tw, th, src8 == mfma::kSrcS8,
-                                      reinterpret_cast<char *>(dst), dstep, tiles_x, v4);
+      const int bands = (S.height + kBoxBand - 1) / kBoxBand;
+      const int cblks = (S.width + 63) / 64;
+      const size_t lds = (size_t)th * 64 * sizeof(int);   // th <= 97: below 64 KiB
+      auto go = [&](auto kernel) {
+         kernel<<<bands * cblks, 64, lds, s>>>(static_cast<const unsigned char *>(S.ptr), S.width,
+                                               S.height, S.step, tw, th,
+                                               reinterpret_cast<char *>(dst), dstep, bands);
+      };
+      const bool sgn = src8 == mfma::kSrcS8;
+      const bool bst = (long long)S.height * dstep < kBoxOob;
+#define OCIP_BOX_CASE(NCH)                                                                      \
+   if (sgn) { if (bst) go(ss_box8<NCH, true, true>); else go(ss_box8<NCH, true, false>); }      \
+   else { if (bst) go(ss_box8<NCH, false, true>); else go(ss_box8<NCH, false, false>); }
+      if (ss_box8_chunks(tw) == 1) {
+         OCIP_BOX_CASE(1)
+      } else {
+         OCIP_BOX_CASE(2)
+      }
+#undef OCIP_BOX_CASE
       launch_check("ss_box8");
    }
    u16x8 *g0 = static_cast<u16x8 *>(work);

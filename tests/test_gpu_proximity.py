@@ -326,9 +326,9 @@ def test_int_mfma_equals_valu_path(ocip, monkeypatch, dtype, shape, tshape):
 
 @pytest.mark.parametrize("dtype", [np.uint8, np.int8])
 def test_int_mfma_pitched_source(ocip, monkeypatch, dtype):
-    """A pitched 8-bit source whose width is not a multiple of 4 (4-byte aligned rows, so the
-    window sums stage whole dwords except where one reaches past the image's right side):
-    the MFMA path equals the VALU path bit for bit."""
+    """A pitched 8-bit source whose width is not a multiple of 4 (the bytes past the width in
+    each row's pitch must read as zero in the window sums and the staged tile): the MFMA path
+    equals the VALU path bit for bit."""
     rng = np.random.default_rng(131)
     info = np.iinfo(dtype)
     big = rng.integers(info.min, info.max, size=(90, 136), endpoint=True).astype(dtype)
