@@ -51,6 +51,37 @@ template <class T> __device__ __forceinline__ T wave_inclusive_scan(T v)
    return v;
 }
 
+// Inclusive scans across the 64 lanes of N doubles at once, by DPP moves of both 32-bit halves
+// (no LDS round trips; the N scans interleaved): within rows of 16 lanes, then rows 1 and 3
+// take the last lane of rows 0 and 2, rows 2 and 3 the last lane of row 1.  A lane without a
+// source adds -0.0, which leaves every value (signed zeros included) unchanged.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+   const unsigned long long b = __double_as_longlong(v);
+   const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROWS, 0xf, false);
+   const int hi = __builtin_amdgcn_update_dpp((int)0x80000000u, (int)(unsigned)(b >> 32), CTRL,
+                                              ROWS, 0xf, false);
+   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+template <int N>
+__device__ __forceinline__ void wave_incl_scan_f64(double (&x)[N])
+{
+#pragma unroll
+   for (int k = 0; k < N; ++k) x[k] += dpp_f64<0x111, 0xf>(x[k]);   // row_shr:1
+#pragma unroll
+   for (int k = 0; k < N; ++k) x[k] += dpp_f64<0x112, 0xf>(x[k]);   // row_shr:2
+#pragma unroll
+   for (int k = 0; k < N; ++k) x[k] += dpp_f64<0x114, 0xf>(x[k]);   // row_shr:4
+#pragma unroll
+   for (int k = 0; k < N; ++k) x[k] += dpp_f64<0x118, 0xf>(x[k]);   // row_shr:8
+#pragma unroll
+   for (int k = 0; k < N; ++k) x[k] += dpp_f64<0x142, 0xa>(x[k]);   // row_bcast:15
+#pragma unroll
+   for (int k = 0; k < N; ++k) x[k] += dpp_f64<0x143, 0xc>(x[k]);   // row_bcast:31
+}
+
 template <class T> __device__ __forceinline__ T wave_sum(T v)
 {
 #pragma unroll

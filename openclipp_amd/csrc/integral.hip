@@ -230,23 +230,29 @@ __global__ __launch_bounds__(kThreads) void integral_final(const char *src, int 
          load_row<TYPE, SQR, VEC>(src, step, y + r, y1, x0, W, nch, ch, v[r]);
          rc[r] = y + r < y1 ? aux.rowsum[(size_t)(y + r) * aux.CB + cb] : 0.0;
       }
+      // the lane's prefixes of the batch's rows, then their wavefront scans together
+      double p[kRowUnroll][kVec], inc[kRowUnroll];
 #pragma unroll
       for (int r = 0; r < kRowUnroll; ++r) {
-         py += rc[r];
-         double p[kVec];
          double loc = 0;
 #pragma unroll
          for (int j = 0; j < kVec; ++j) {
             colrun[j] += v[r][j];
             loc += colrun[j];
-            p[j] = loc;
+            p[r][j] = loc;
          }
-         const double base = corner + py + (wave_inclusive_scan(loc) - loc);
+         inc[r] = loc;
+      }
+      wave_incl_scan_f64<kRowUnroll>(inc);
+#pragma unroll
+      for (int r = 0; r < kRowUnroll; ++r) {
+         py += rc[r];
+         const double base = corner + py + (inc[r] - p[r][kVec - 1]);
          const int yy = y + r;
          if (yy >= y1) continue;
          double val[kVec];
 #pragma unroll
-         for (int j = 0; j < kVec; ++j) val[j] = base + p[j] + px[j];
+         for (int j = 0; j < kVec; ++j) val[j] = base + p[r][j] + px[j];
          if (OUT == 0)
             store4<float, OVEC>(reinterpret_cast<float *>(out + (size_t)yy * ostep), x0, W, val);
          else if (OUT == 1)
