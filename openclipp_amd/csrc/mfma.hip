@@ -815,11 +815,14 @@ void launch_crosscorr_mfma(hipStream_t s, int precision, bool norm, const View &
 
 // ======================================================================= many sources
 // Split-BF16 CrossCorr maps (anchor (tw/2, th/2), zero outside the image, no Norm) of several
-// sources of one size with one template: crosscorr_mfma with 2 column tiles, the
-// images of up to kZImages sources in one launch.  The template's fragments are prepared once
-// by prep_crosscorr_split.
+// sources of one size with one template: crosscorr_mfma (NT = 2 or 4) or the row-pair kernel
+// (NT = 4 or 6), the autotuner's choice, the images of up to kZImages sources in one launch.
+// The template's fragments, in both layouts, are prepared once by prep_crosscorr_split.
 static_assert(kSplitImages == mfma::kZImages, "sources per launch");
-size_t crosscorr_split_frag_bytes(int tw, int th) { return mfma_frag_bytes(mfma::kBF16x3, tw, th); }
+size_t crosscorr_split_frag_bytes(int tw, int th)
+{
+   return mfma_frag_bytes(mfma::kBF16x3, tw, th) + pair_frag_bytes(mfma::kBF16x3, tw, th);
+}
 
 // rows per LDS band for the wider of the two tile widths the autotuner tries (NT = 4)
 static Geometry split_geometry(int tw, int th, int NT = 4)
@@ -831,25 +834,39 @@ static Geometry split_geometry(int tw, int th, int NT = 4)
 
 void prep_crosscorr_split(hipStream_t s, const View &T, void *frags, double *sums)
 {
-   const int tw = T.width, th = T.height, KC = kc_for(tw), nub = nub_for(tw);
+   const int tw = T.width, th = T.height, KC = kc_for(tw), KC2 = kc2_for(tw), nub = nub_for(tw);
    (void)split_geometry(tw, th);
    u16x8 *f0 = static_cast<u16x8 *>(frags);
    const BatchDesc d{static_cast<const char *>(T.ptr), tw, th, T.step, KC, nub, f0,
                      f0 + (size_t)nub * th * KC * 64, sums};
    mfma::launch_single_prep<mfma::kBF16x3>(s, d);
+   // the row-pair layout after it (both preps write the same sum of squares)
+   u16x8 *g0 = reinterpret_cast<u16x8 *>(static_cast<char *>(frags) +
+                                         mfma_frag_bytes(mfma::kBF16x3, tw, th));
+   const BatchDesc e{static_cast<const char *>(T.ptr), tw, th, T.step, KC2, nub, g0,
+                     g0 + pair_frag_count(tw, th) * 64, sums};
+   mfma::launch_pair_prep<mfma::kBF16x3>(s, e);
 }
 
 void launch_crosscorr_split(hipStream_t s, int n, const float *const *src, const int *sstep,
                             float *const *dst, int dstep, int W, int H, int tw, int th,
                             const void *frags, const double *sums)
 {
-   const int KC = kc_for(tw), nub = nub_for(tw);
+   const int KC = kc_for(tw), KC2 = kc2_for(tw), nub = nub_for(tw);
    const int vb = split_geometry(tw, th).vb;
    const u16x8 *f0 = static_cast<const u16x8 *>(frags);
    const u16x8 *f1 = f0 + (size_t)nub * th * KC * 64;
-   // one launch per kZImages sources, with NT = 2 or 4 column tiles per workgroup
-   auto run = [&](int NT) {
-      const Geometry g = split_geometry(tw, th, NT);
+   const u16x8 *g0 = reinterpret_cast<const u16x8 *>(static_cast<const char *>(frags) +
+                                                     mfma_frag_bytes(mfma::kBF16x3, tw, th));
+   const u16x8 *g1 = g0 + pair_frag_count(tw, th) * 64;
+   // one launch per kZImages sources: variant 2 or 4, crosscorr_mfma with NT = 2 or 4 column
+   // tiles per workgroup; 14 or 16, the row-pair kernel with NT = 4 or 6
+   auto run = [&](int variant) {
+      const bool pair = variant > 10;
+      const int NT = pair ? variant - 10 : variant;
+      const Geometry g = pair ? pair_geometry(KC2, NT, th, mfma::kLdsMax, 2)
+                              : split_geometry(tw, th, NT);
+      if (g.vb < 1) fail(CL_INVALID_VALUE, "template too large for the MFMA tile");
       for (int i0 = 0; i0 < n; i0 += mfma::kZImages) {
          SingleLaunch L;
          L.z.n = std::min(mfma::kZImages, n - i0);
@@ -862,14 +879,22 @@ void launch_crosscorr_split(hipStream_t s, int n, const float *const *src, const
             if (reinterpret_cast<uintptr_t>(src[i0 + k]) % 16 || sstep[i0 + k] % 16) vec = 0;
          }
          L.mode = mfma::kModeCC;
-         L.a = mfma::TileArgs{src[i0], W, H, sstep[i0], tw, th, nub, f0, f1, sums, nullptr,
-                              dst[i0], dstep, vb, vec};
-         L.KC = KC;
          L.NT = NT;
          L.ldsp = g.ldsp;
          L.lds = g.lds;
-         L.grid = dim3((W + 32 * NT - 1) / (32 * NT), (H + mfma::kTileRows - 1) / mfma::kTileRows);
-         mfma::launch_single<mfma::kBF16x3>(s, L);
+         if (pair) {
+            L.a = mfma::TileArgs{src[i0], W, H, sstep[i0], tw, th, nub, g0, g1, sums, nullptr,
+                                 dst[i0], dstep, g.vb, vec};
+            L.KC = KC2;
+            L.grid = dim3((W + 16 * NT - 1) / (16 * NT), (H + mfma::kPairRows - 1) / mfma::kPairRows);
+            mfma::launch_pair<mfma::kBF16x3>(s, L);
+         } else {
+            L.a = mfma::TileArgs{src[i0], W, H, sstep[i0], tw, th, nub, f0, f1, sums, nullptr,
+                                 dst[i0], dstep, vb, vec};
+            L.KC = KC;
+            L.grid = dim3((W + 32 * NT - 1) / (32 * NT), (H + mfma::kTileRows - 1) / mfma::kTileRows);
+            mfma::launch_single<mfma::kBF16x3>(s, L);
+         }
       }
    };
    // the autotuner's choice for this shape and number of sources (each candidate rewrites the
@@ -877,10 +902,10 @@ void launch_crosscorr_split(hipStream_t s, int n, const float *const *src, const
    char shape[64];
    snprintf(shape, sizeof shape, "tw%d:th%d:n%d", tw, th, std::min(n, (int)mfma::kZImages));
    const View v0{src[0], W, H, sstep[0], 1, SImage::F32};
-   const int NT = tuned_variant(tune_key("split", mfma::kBF16x3, mfma::kModeCC, shape, v0), s, run,
-                                {2, 4});
+   const int variant = tuned_variant(tune_key("split", mfma::kBF16x3, mfma::kModeCC, shape, v0), s,
+                                     run, {16, 14, 4, 2});
    KernelTimer kt(s, "crosscorr_mfma");
-   run(NT);
+   run(variant);
 }
 
 // ======================================================================= exact integer sums

@@ -41,7 +41,7 @@ template <int KC, int NT, int MODE> void pair(hipStream_t s, const SingleLaunch 
    auto k = crosscorr_mfma_pair<P, KC, NT, MODE>;
    allow_lds(k);
    const int tiles = L.grid.x * L.grid.y;
-   k<<<tiles, 256, L.lds, s>>>(L.a, L.ldsp, (int)L.grid.x, tiles);
+   k<<<tiles * std::max(1, L.z.n), 256, L.lds, s>>>(L.a, L.ldsp, (int)L.grid.x, tiles, L.z);
 }
 
 template <int KC> void pair_mode(hipStream_t s, const SingleLaunch &L)

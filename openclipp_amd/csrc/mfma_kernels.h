@@ -920,14 +920,22 @@ __host__ __device__ constexpr int pair_waves(int P, int KC, int NT, int MODE)
    return NT <= 4 && MODE == kModeCC && P != kBF16x3 ? 3 : P == kBF16x3 && NT == 6 && KC >= 7 ? 1 : 2;
 }
 
-// One 127 x 16 NT tile per workgroup, XCD-aware order (the single-template call).
+// One 127 x 16 NT tile per workgroup, XCD-aware order (the single-template call): tiles_x *
+// tiles_y workgroups per image, images z.n of them one after the other (or the one image of
+// `a`).
 template <int P, int KC, int NT, int MODE>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(pair_waves(P, KC, NT, MODE), pair_waves(P, KC, NT, MODE))))
-void crosscorr_mfma_pair(TileArgs a, int ldsp, int tiles_x, int tiles)
+void crosscorr_mfma_pair(TileArgs a, int ldsp, int tiles_x, int tiles, ZImages z)
 {
    extern __shared__ u16x8 lds_v[];
-   const int t = xcd_tile(blockIdx.x, tiles);
+   const int zi = blockIdx.x / tiles;
+   const int t = xcd_tile(blockIdx.x - zi * tiles, tiles);
+   if (z.n) {
+      a.src = z.src[zi];
+      a.dst = z.dst[zi];
+      a.sstep = z.sstep[zi];
+   }
    pair_tile<P, KC, NT, MODE>(a, t % tiles_x, t / tiles_x, reinterpret_cast<unsigned short *>(lds_v),
                               ldsp);
 }
