@@ -321,6 +321,40 @@ template <int OP, int TYPE> __device__ __forceinline__ int32_t itap4(int32_t acc
    return (int32_t)__builtin_amdgcn_udot4(s, t, (uint32_t)acc, false);
 }
 
+// AbsDistance on 8-bit sources: the staged tile's keyed values (sad_key) repacked in place as
+// P(c) = bytes (v[c], v[c + 4], v[c + 8], v[c + 12]) for the columns c < ncols (the tile was
+// staged ncols + 12 columns wide, within the pitch), so that the 16 taps u0 .. u0 + 15 of the
+// output at window column x are four v_sad_u8 of P(x + k) against the template bytes
+// (t[u0 + k], t[u0 + k + 4], t[u0 + k + 8], t[u0 + k + 12]), k < 4, with no byte
+// realignment.  Rounds of 8 elements per thread in increasing order: every read of a round
+// (elements i .. i + 12) precedes its writes (elements i) at a barrier, and later rounds read
+// only elements no earlier round wrote.
+template <int TYPE>
+__device__ __forceinline__ void pack_sad8(int32_t *lds, int ldsw, int nrows, int ncols)
+{
+   const int n = nrows * ldsw;
+   for (int i0 = 0; i0 < n; i0 += 256 * 8) {
+      uint32_t p[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+         const int i = i0 + threadIdx.x + 256 * k;
+         const int c = i % ldsw;
+         p[k] = 0;
+         if (i < n && c < ncols)
+            p[k] = pack4(sad_key<kSAD, TYPE, int32_t>(lds[i]), sad_key<kSAD, TYPE, int32_t>(lds[i + 4]),
+                         sad_key<kSAD, TYPE, int32_t>(lds[i + 8]),
+                         sad_key<kSAD, TYPE, int32_t>(lds[i + 12]));
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+         const int i = i0 + threadIdx.x + 256 * k;
+         if (i < n && i % ldsw < ncols) lds[i] = (int32_t)p[k];
+      }
+      __syncthreads();
+   }
+}
+
 // With int32 sums (8-bit sources) SqrDistance and SqrDistance_Norm take the cross term only,
 // one multiply-add per tap, and finish as sum s^2 - 2 sum s*t + sum t^2: the window's sum of
 // squares from the FP64 integral (an exact integer below 2^53), sum t^2 from tsums[0].
@@ -344,6 +378,9 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
    // 8-bit sources with int32 sums: four taps per instruction on packed bytes
    constexpr bool kPacked = sizeof(ACC) == 4 && (TYPE == SImage::U8 || TYPE == SImage::S8) &&
                             (TOP == kCC || TOP == kCCNorm || TOP == kSAD);
+   // AbsDistance on 8-bit sources: the tile repacked by pack_sad8 (the host sized the pitch
+   // for ncols + 12 staged columns)
+   constexpr bool kSad8 = kPacked && TOP == kSAD;
    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
    const int x0 = blockIdx.x * (kWaves * R), y0 = blockIdx.y * kTileRows;
    const int cx = tw / 2, cy = th / 2;
@@ -357,12 +394,48 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
    for (int v0 = 0; v0 < th; v0 += vb) {
       const int nv = min(vb, th - v0);
       __syncthreads();
-      stage_tile_int<TYPE>(ldsi, ldsw, kTileRows + nv - 1, ncols, src, W, H, sstep, nch, ch,
-                           x0 - cx, y0 - cy + v0);
+      stage_tile_int<TYPE>(ldsi, ldsw, kTileRows + nv - 1, kSad8 ? ncols + 12 : ncols, src, W,
+                           H, sstep, nch, ch, x0 - cx, y0 - cy + v0);
       __syncthreads();
+      if constexpr (kSad8) pack_sad8<TYPE>(ldsi, ldsw, kTileRows + nv - 1, ncols);
       for (int v = 0; v < nv; ++v) {
          const int32_t *lrow = ldsi + (size_t)(lane + v) * ldsw + colbase;
          const int32_t *trow = tp + (size_t)(v0 + v) * tpitch;
+         if constexpr (kSad8) {
+            // 16 taps per step: the window's P values of columns u0 .. u0 + R + 2 (16-byte
+            // reads from a 16-byte aligned column: ldsw, colbase and u0 are multiples of 4),
+            // four v_sad_u8 per output; the last tw % 16 taps one at a time from byte 0 of P
+            // (the keyed value of that column)
+            int u0 = 0;
+            for (; u0 + 16 <= tw; u0 += 16) {
+               uint32_t pw[R + 4];
+               const int4 *w4 = reinterpret_cast<const int4 *>(lrow + u0);
+#pragma unroll
+               for (int m = 0; m < (R + 4) / 4; ++m) {
+                  const int4 q4 = w4[m];
+                  pw[4 * m] = (uint32_t)q4.x; pw[4 * m + 1] = (uint32_t)q4.y;
+                  pw[4 * m + 2] = (uint32_t)q4.z; pw[4 * m + 3] = (uint32_t)q4.w;
+               }
+               uint32_t tq[4];
+#pragma unroll
+               for (int k = 0; k < 4; ++k)
+                  tq[k] = pack4(sad_key<kSAD, TYPE, ACC>(trow[u0 + k]),
+                                sad_key<kSAD, TYPE, ACC>(trow[u0 + k + 4]),
+                                sad_key<kSAD, TYPE, ACC>(trow[u0 + k + 8]),
+                                sad_key<kSAD, TYPE, ACC>(trow[u0 + k + 12]));
+#pragma unroll
+               for (int r = 0; r < R; ++r)
+#pragma unroll
+                  for (int k = 0; k < 4; ++k)
+                     acc[r] = (ACC)__builtin_amdgcn_sad_u8(pw[r + k], tq[k], (uint32_t)acc[r]);
+            }
+            for (; u0 < tw; ++u0) {
+               const int32_t t = sad_key<kSAD, TYPE, ACC>(trow[u0]);
+#pragma unroll
+               for (int r = 0; r < R; ++r) acc[r] = itap<kSAD, ACC>(acc[r], lrow[u0 + r] & 0xff, t);
+            }
+            continue;
+         }
          // the template row in chunks of kIntU columns: kIntU template values and the
          // R + kIntU - 1 window values they meet, then R x kIntU taps (exact integer sums,
          // so the order is free); LDS columns used: colbase .. colbase + R + tw - 2 (the
@@ -564,7 +637,15 @@ static void launch_op(hipStream_t s, const View &S, const View &T, void *dst, in
                           S.type == SImage::U16 || S.type == SImage::S16;
    const int R = tile_r(W, tw, th);
    const int ncols = kWaves * R + tw - 1;
-   const int ldsw = lds_pitch(ncols);
+   // int32 accumulation is exact when taps * max|term| < 2^31: 8-bit sources up to 33025
+   // taps, and AbsDistance on 16-bit sources up to 32768 taps; the other 16-bit cases
+   // accumulate in int64.
+   const bool bits8 = S.type == SImage::U8 || S.type == SImage::S8;
+   const long long taps = (long long)tw * th;
+   const bool fits32 = bits8 ? taps * 65025LL < (1LL << 31)
+                             : OP == kSAD && taps * 65535LL < (1LL << 31);
+   // AbsDistance on 8-bit sources with int32 sums stages 12 more columns (pack_sad8)
+   const int ldsw = lds_pitch(ncols + (OP == kSAD && bits8 && fits32 ? 12 : 0));
    const int rows_fit = kLdsBudget / (ldsw * 4) - (kTileRows - 1);
    const bool tiled = S.type != SImage::F64 && rows_fit >= 1 && th > 0 && tw > 0;
 
@@ -612,13 +693,6 @@ static void launch_op(hipStream_t s, const View &S, const View &T, void *dst, in
 
    if (small_int) {
       KernelTimer kt(s, "prox_tile_int");
-      // int32 accumulation is exact when taps * max|term| < 2^31: 8-bit sources up to 33025
-      // taps, and AbsDistance on 16-bit sources up to 32768 taps; the other 16-bit cases
-      // accumulate in int64.
-      const bool bits8 = S.type == SImage::U8 || S.type == SImage::S8;
-      const long long taps = (long long)tw * th;
-      const bool fits32 = bits8 ? taps * 65025LL < (1LL << 31)
-                                : OP == kSAD && taps * 65535LL < (1LL << 31);
       OCIP_TYPE_SWITCH(S.type, TT, {
          if (fits32) {
             if (R == 16) OCIP_TILE_INT(TT, 16, int32_t); else OCIP_TILE_INT(TT, 8, int32_t);
