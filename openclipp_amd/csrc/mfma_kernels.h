@@ -296,6 +296,56 @@ __device__ __forceinline__ void stage_rows(const TileArgs &a, unsigned short *ld
    constexpr int groups = LCOLS / 8;
    const int W = a.W, H = a.H;
    const int total = nrows * groups;
+   if constexpr (SRC8) {
+      // 8-bit sources: eight items per thread in flight, each held as its raw bytes (two
+      // dwords) until every load of the batch is out
+      constexpr int K = 8;
+      for (int it0 = threadIdx.x; it0 < total; it0 += 256 * K) {
+         uint32_t lo[K], hi[K];
+#pragma unroll
+         for (int k = 0; k < K; ++k) {
+            const int it = it0 + 256 * k;
+            const int r = it / groups, g = it - r * groups;
+            const int sy = row0 + r, sx = col0 + 8 * g;
+            const bool rowin = it < total && sy >= 0 && sy < H;
+            const unsigned char *brow = reinterpret_cast<const unsigned char *>(a.src) +
+                                        (size_t)(rowin ? sy : 0) * a.sstep;
+            if (a.vec && rowin && sx >= 0 && sx + 8 <= W) {
+               // a.vec: 8-byte aligned rows and staged columns starting on multiples of 8
+               const uint2 q = *reinterpret_cast<const uint2 *>(brow + sx);
+               lo[k] = q.x;
+               hi[k] = q.y;
+            } else {
+               lo[k] = hi[k] = 0;
+#pragma unroll
+               for (int i = 0; i < 8; ++i) {
+                  const int x = sx + i;
+                  const uint32_t b = (rowin && x >= 0 && x < W) ? brow[x] : 0u;
+                  if (i < 4) lo[k] |= b << (8 * i); else hi[k] |= b << (8 * (i - 4));
+               }
+            }
+         }
+#pragma unroll
+         for (int k = 0; k < K; ++k) {
+            const int it = it0 + 256 * k;
+            if (it >= total) break;
+            const int r = it / groups, g = it - r * groups;
+            u16x8 h0, h1;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+               const unsigned b = ((i < 4 ? lo[k] : hi[k]) >> (8 * (i & 3))) & 0xffu;
+               const float f = a.src8 == kSrcS8 ? (float)(int)(signed char)b : (float)b;
+               unsigned short e0, e1;
+               encode<P>(f, e0, e1);
+               h0[i] = e0;
+               h1[i] = e1;
+            }
+            *reinterpret_cast<u16x8 *>(lds + (size_t)r * ldsp + 8 * g) = h0;
+            if (NPL == 2) *reinterpret_cast<u16x8 *>(lds + plane + (size_t)r * ldsp + 8 * g) = h1;
+         }
+      }
+      return;
+   }
    for (int it0 = threadIdx.x; it0 < total; it0 += 256 * 4) {
       float f[4][8];
 #pragma unroll
@@ -304,25 +354,6 @@ __device__ __forceinline__ void stage_rows(const TileArgs &a, unsigned short *ld
          const int r = it / groups, g = it - r * groups;
          const int sy = row0 + r, sx = col0 + 8 * g;
          const bool rowin = it < total && sy >= 0 && sy < H;
-         if constexpr (SRC8) {
-            const char *brow = reinterpret_cast<const char *>(a.src) + (size_t)(rowin ? sy : 0) * a.sstep;
-            if (a.vec && rowin && sx >= 0 && sx + 8 <= W) {
-               // a.vec: 8-byte aligned rows and staged columns starting on multiples of 8
-               const uint2 q = *reinterpret_cast<const uint2 *>(brow + sx);
-#pragma unroll
-               for (int i = 0; i < 8; ++i) {
-                  const unsigned b = ((i < 4 ? q.x : q.y) >> (8 * (i & 3))) & 0xffu;
-                  f[k][i] = a.src8 == kSrcS8 ? (float)(int)(signed char)b : (float)b;
-               }
-            } else {
-#pragma unroll
-               for (int i = 0; i < 8; ++i) {
-                  const int x = sx + i;
-                  f[k][i] = (rowin && x >= 0 && x < W) ? tmpl_elem(brow, x, a.src8) : 0.f;
-               }
-            }
-            continue;
-         }
          const float *srow = reinterpret_cast<const float *>(
              reinterpret_cast<const char *>(a.src) + (size_t)(rowin ? sy : 0) * a.sstep);
          if (a.vec && rowin && sx >= 0 && sx + 8 <= W) {
