@@ -229,6 +229,9 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) float gmem_f32;
 typedef __attribute__((address_space(1))) f32x4_t gmem_f32x4;
 typedef __attribute__((address_space(1))) char gmem_char;
+typedef __attribute__((address_space(1))) int gmem_i32;
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) i32x4_t gmem_i32x4;
 __device__ __forceinline__ gmem_f32 *out_row(const TileArgs &a, int y)
 {
    return reinterpret_cast<gmem_f32 *>((gmem_char *)a.dst + (size_t)y * a.dstep);
@@ -795,6 +798,33 @@ __device__ __forceinline__ void pair_tile(const TileArgs &a, int bx, int by, uns
    };
    Corners qa, qb;
    if constexpr (NORM) load_corners(0, qa);
+   // kModeInt: the window sums of s^2 that ss_box8 left in the map (all ops but CrossCorr),
+   // the thread's groups of four outputs in the epilogue's order, loaded before the halves
+   // are paired so that their latency passes under it
+   constexpr int kIntIters = (kPairRows * (C / 4) + 255) / 256;
+   int4 ssq[INT ? kIntIters : 1];
+   if constexpr (INT) {
+      const bool v4 = a.dstep % 16 == 0 && reinterpret_cast<uintptr_t>(a.dst) % 16 == 0;
+#pragma unroll
+      for (int i = 0; i < kIntIters; ++i) {
+         const int k = threadIdx.x + 256 * i;
+         const int r = k / (C / 4), c = 4 * (k - r * (C / 4));
+         const int y = Y0 + r, x = x0 + c;
+         ssq[i] = make_int4(0, 0, 0, 0);
+         if (a.op == kCC || k >= kPairRows * (C / 4) || y >= a.H || x >= a.W) continue;
+         const gmem_i32 *oi = reinterpret_cast<const gmem_i32 *>(out_row(a, y) + x);
+         if (v4 && x + 4 <= a.W) {
+            const i32x4_t q = *reinterpret_cast<const gmem_i32x4 *>(oi);
+            ssq[i] = make_int4(q[0], q[1], q[2], q[3]);
+         } else {
+            int e4[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+               if (x + e < a.W) e4[e] = oi[e];
+            ssq[i] = make_int4(e4[0], e4[1], e4[2], e4[3]);
+         }
+      }
+   }
 
    // Pair the halves in LDS: tile row r = 32 w + m + dy holds output row Y0 - 1 + r.
    // C fragment: lane holds column n = lane % 32 of each n-tile, rows
@@ -831,25 +861,16 @@ __device__ __forceinline__ void pair_tile(const TileArgs &a, int bx, int by, uns
       // it; four consecutive columns per thread, one 16-byte store where the row allows
       const double tt = a.tsums[0];
       const bool v4 = a.dstep % 16 == 0 && reinterpret_cast<uintptr_t>(a.dst) % 16 == 0;
-      for (int k = threadIdx.x; k < kPairRows * (C / 4); k += 256) {
+#pragma unroll
+      for (int i = 0; i < kIntIters; ++i) {
+         const int k = threadIdx.x + 256 * i;
          const int r = k / (C / 4), c = 4 * (k - r * (C / 4));
          const int y = Y0 + r, x = x0 + c;
-         if (y >= a.H || x >= a.W) continue;
+         if (k >= kPairRows * (C / 4) || y >= a.H || x >= a.W) continue;
          const int4 cc = *reinterpret_cast<const int4 *>(&itile[(r + 1) * TP + c]);
          const int ci[4] = {cc.x, cc.y, cc.z, cc.w};
-         float *o = reinterpret_cast<float *>(reinterpret_cast<char *>(a.dst) + (size_t)y * a.dstep) + x;
-         const int *oi = reinterpret_cast<const int *>(o);
-         int si[4] = {0, 0, 0, 0};
-         if (a.op != kCC) {
-            if (v4 && x + 4 <= a.W) {
-               const int4 q = *reinterpret_cast<const int4 *>(oi);
-               si[0] = q.x; si[1] = q.y; si[2] = q.z; si[3] = q.w;
-            } else {
-#pragma unroll
-               for (int e = 0; e < 4; ++e)
-                  if (x + e < a.W) si[e] = oi[e];
-            }
-         }
+         gmem_f32 *o = out_row(a, y) + x;
+         const int si[4] = {ssq[i].x, ssq[i].y, ssq[i].z, ssq[i].w};
          float v[4];
 #pragma unroll
          for (int e = 0; e < 4; ++e) {
@@ -863,7 +884,7 @@ __device__ __forceinline__ void pair_tile(const TileArgs &a, int bx, int by, uns
             }
          }
          if (v4 && x + 4 <= a.W) {
-            *reinterpret_cast<float4 *>(o) = make_float4(v[0], v[1], v[2], v[3]);
+            *reinterpret_cast<gmem_f32x4 *>(o) = f32x4_t{v[0], v[1], v[2], v[3]};
          } else {
 #pragma unroll
             for (int e = 0; e < 4; ++e)
