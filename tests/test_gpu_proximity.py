@@ -297,12 +297,13 @@ def test_integer_4096_float_maps_reference_order(ocip):
 
 
 # (the window sums of s^2 take one 128-column chunk per row up to tw = 64 and two beyond, a
-# ring of th rows, 64-row bands of 64 columns: shapes across each of these boundaries, and a
-# template nearly as large as the image)
+# ring of th rows, 64-row bands of 64 columns: shapes across each of these boundaries, images
+# whose last block holds one column, and a template nearly as large as the image)
 # (image rows, columns), (template rows, columns)
 INT_SHAPES = [((37, 45), (1, 1)), ((70, 90), (3, 5)), ((200, 260), (32, 32)),
               ((150, 300), (97, 40)), ((300, 130), (16, 97)), ((129, 250), (64, 64)),
-              ((260, 140), (31, 8)), ((52, 70), (50, 65)), ((70, 129), (1, 66))]
+              ((260, 140), (31, 8)), ((52, 70), (50, 65)), ((70, 129), (1, 66)),
+              ((90, 257), (5, 97)), ((130, 385), (33, 2))]
 
 
 @pytest.mark.parametrize("dtype", [np.uint8, np.int8])
