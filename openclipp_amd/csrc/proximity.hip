@@ -402,6 +402,26 @@ __device__ __forceinline__ void pack_sad8(int32_t *lds, int ldsw, int nrows, int
    }
 }
 
+// AbsDistance on 8-bit sources: the staged template (int32 per column, row pitch tpitch) gets,
+// in each full block of 16 columns, the four dwords of the block's keyed bytes that the
+// repacked tile meets, in place at the block's first four columns: dword k = bytes of columns
+// 16 b + k, + 4, + 8, + 12.  The columns of a last partial block keep their values.  One
+// thread per (row, block).
+template <int TYPE>
+__global__ __launch_bounds__(256) void sad8_template_pack(int32_t *tp, int tpitch, int tw, int th)
+{
+   const int nb = tw / 16;
+   const int i = blockIdx.x * 256 + threadIdx.x;
+   if (i >= nb * th) return;
+   const int v = i / nb, b = i - v * nb;
+   int32_t *blk = tp + (size_t)v * tpitch + 16 * b;
+   int32_t t[16];
+#pragma unroll
+   for (int k = 0; k < 16; ++k) t[k] = sad_key<kSAD, TYPE, int32_t>(blk[k]);
+#pragma unroll
+   for (int k = 0; k < 4; ++k) blk[k] = (int32_t)pack4(t[k], t[k + 4], t[k + 8], t[k + 12]);
+}
+
 // With int32 sums (8-bit sources) SqrDistance and SqrDistance_Norm take the cross term only,
 // one multiply-add per tap, and finish as sum s^2 - 2 sum s*t + sum t^2: the window's sum of
 // squares from the FP64 integral (an exact integer below 2^53), sum t^2 from tsums[0].
@@ -426,7 +446,8 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
    constexpr bool kPacked = sizeof(ACC) == 4 && (TYPE == SImage::U8 || TYPE == SImage::S8) &&
                             (TOP == kCC || TOP == kCCNorm || TOP == kSAD);
    // AbsDistance on 8-bit sources: the tile repacked by pack_sad8 (the host sized the pitch
-   // for ncols + 12 staged columns)
+   // for ncols + 12 staged columns), the template's full 16-column blocks by
+   // sad8_template_pack
    constexpr bool kSad8 = kPacked && TOP == kSAD;
    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
    const int x0 = blockIdx.x * (kWaves * R), y0 = blockIdx.y * kTileRows;
@@ -456,6 +477,10 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
             int u0 = 0;
             for (; u0 + 16 <= tw; u0 += 16) {
                uint32_t pw[R + 4];
+               // (an empty asm with a memory clobber: each step reads its window afresh as 16-byte
+               // loads, rather than the compiler carrying one dword over from the step before
+               // and reading the rest as conflicting 4-byte loads)
+               asm volatile("" ::: "memory");
                const int4 *w4 = reinterpret_cast<const int4 *>(lrow + u0);
 #pragma unroll
                for (int m = 0; m < (R + 4) / 4; ++m) {
@@ -463,13 +488,10 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
                   pw[4 * m] = (uint32_t)q4.x; pw[4 * m + 1] = (uint32_t)q4.y;
                   pw[4 * m + 2] = (uint32_t)q4.z; pw[4 * m + 3] = (uint32_t)q4.w;
                }
+               // the block's four packed template dwords (sad8_template_pack)
                uint32_t tq[4];
 #pragma unroll
-               for (int k = 0; k < 4; ++k)
-                  tq[k] = pack4(sad_key<kSAD, TYPE, ACC>(trow[u0 + k]),
-                                sad_key<kSAD, TYPE, ACC>(trow[u0 + k + 4]),
-                                sad_key<kSAD, TYPE, ACC>(trow[u0 + k + 8]),
-                                sad_key<kSAD, TYPE, ACC>(trow[u0 + k + 12]));
+               for (int k = 0; k < 4; ++k) tq[k] = (uint32_t)trow[u0 + k];
 #pragma unroll
                for (int r = 0; r < R; ++r)
 #pragma unroll
@@ -739,6 +761,18 @@ static void launch_op(hipStream_t s, const View &S, const View &T, void *dst, in
        tsums, integ, static_cast<char *>(dst), dstep, vb, ldsw)
 
    if (small_int) {
+      if (OP == kSAD && bits8 && fits32 && tw >= 16) {
+         // the packed template blocks of the tile's AbsDistance loop (in the staged template,
+         // which the next call stages again)
+         KernelTimer kt(s, "template_prep");
+         const int items = (tw / 16) * th;
+         int32_t *tp = static_cast<int32_t *>(const_cast<void *>(tstage));
+         OCIP_TYPE_SWITCH(S.type, TT, {
+            if constexpr (TT == SImage::U8 || TT == SImage::S8)
+               sad8_template_pack<TT><<<(items + 255) / 256, 256, 0, s>>>(tp, tpitch, tw, th);
+         });
+         launch_check("sad8_template_pack");
+      }
       KernelTimer kt(s, "prox_tile_int");
       OCIP_TYPE_SWITCH(S.type, TT, {
          if (fits32) {
