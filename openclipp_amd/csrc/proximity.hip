@@ -403,21 +403,23 @@ __device__ __forceinline__ void pack_sad8(int32_t *lds, int ldsw, int nrows, int
 }
 
 // AbsDistance on 8-bit sources: the staged template (int32 per column, row pitch tpitch) gets,
-// in each full block of 16 columns, the four dwords of the block's keyed bytes that the
-// repacked tile meets, in place at the block's first four columns: dword k = bytes of columns
-// 16 b + k, + 4, + 8, + 12.  The columns of a last partial block keep their values.  One
+// in each block of 16 columns, the four dwords of the block's keyed bytes that the repacked
+// tile meets, in place at the block's first four columns: dword k = bytes of columns 16 b + k,
+// + 4, + 8, + 12, a zero byte for a column past the template (a last partial block).  One
 // thread per (row, block).
+__host__ __device__ constexpr int sad8_blocks(int tw) { return (tw + 15) / 16; }
+
 template <int TYPE>
 __global__ __launch_bounds__(256) void sad8_template_pack(int32_t *tp, int tpitch, int tw, int th)
 {
-   const int nb = tw / 16;
+   const int nb = sad8_blocks(tw);
    const int i = blockIdx.x * 256 + threadIdx.x;
    if (i >= nb * th) return;
    const int v = i / nb, b = i - v * nb;
    int32_t *blk = tp + (size_t)v * tpitch + 16 * b;
    int32_t t[16];
 #pragma unroll
-   for (int k = 0; k < 16; ++k) t[k] = sad_key<kSAD, TYPE, int32_t>(blk[k]);
+   for (int k = 0; k < 16; ++k) t[k] = 16 * b + k < tw ? sad_key<kSAD, TYPE, int32_t>(blk[k]) : 0;
 #pragma unroll
    for (int k = 0; k < 4; ++k) blk[k] = (int32_t)pack4(t[k], t[k + 4], t[k + 8], t[k + 12]);
 }
@@ -472,8 +474,8 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
          if constexpr (kSad8) {
             // 16 taps per step: the window's P values of columns u0 .. u0 + R + 2 (16-byte
             // reads from a 16-byte aligned column: ldsw, colbase and u0 are multiples of 4),
-            // four v_sad_u8 per output; the last tw % 16 taps one at a time from byte 0 of P
-            // (the keyed value of that column)
+            // four v_sad_u8 per output; the last tw % 16 taps in one masked step of the same
+            // kind
             int u0 = 0;
             for (; u0 + 16 <= tw; u0 += 16) {
                uint32_t pw[R + 4];
@@ -498,10 +500,37 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
                   for (int k = 0; k < 4; ++k)
                      acc[r] = (ACC)__builtin_amdgcn_sad_u8(pw[r + k], tq[k], (uint32_t)acc[r]);
             }
-            for (; u0 < tw; ++u0) {
-               const int32_t t = sad_key<kSAD, TYPE, ACC>(trow[u0]);
+            if (u0 < tw) {
+               // a last block of 1 to 15 taps: the same step, each P masked to the bytes of
+               // taps inside the template (its packed template dword holds zeros past it), so
+               // that every other byte adds |0 - 0|, and the dwords k >= rem, which hold no
+               // tap, skipped.  The window reads stay inside the row pitch: columns up to
+               // colbase + u0 + 19 <= ncols + 3.
+               const int rem = tw - u0;
+               asm volatile("" ::: "memory");
+               uint32_t pw[R + 4];
+               const int4 *w4 = reinterpret_cast<const int4 *>(lrow + u0);
 #pragma unroll
-               for (int r = 0; r < R; ++r) acc[r] = itap<kSAD, ACC>(acc[r], lrow[u0 + r] & 0xff, t);
+               for (int m = 0; m < (R + 4) / 4; ++m) {
+                  const int4 q4 = w4[m];
+                  pw[4 * m] = (uint32_t)q4.x; pw[4 * m + 1] = (uint32_t)q4.y;
+                  pw[4 * m + 2] = (uint32_t)q4.z; pw[4 * m + 3] = (uint32_t)q4.w;
+               }
+               uint32_t tq[4], mk[4];
+#pragma unroll
+               for (int k = 0; k < 4; ++k) {
+                  tq[k] = (uint32_t)trow[u0 + k];
+                  // bytes j of tap u0 + k + 4 j < tw: (rem - k + 3) / 4 of them
+                  const int nbytes = (rem - k + 3) >> 2;
+                  mk[k] = nbytes >= 4 ? 0xffffffffu : (1u << (8 * nbytes)) - 1u;
+               }
+#pragma unroll
+               for (int k = 0; k < 4; ++k) {
+                  if (k >= rem) break;
+#pragma unroll
+                  for (int r = 0; r < R; ++r)
+                     acc[r] = (ACC)__builtin_amdgcn_sad_u8(pw[r + k] & mk[k], tq[k], (uint32_t)acc[r]);
+               }
             }
             continue;
          }
@@ -761,11 +790,11 @@ static void launch_op(hipStream_t s, const View &S, const View &T, void *dst, in
        tsums, integ, static_cast<char *>(dst), dstep, vb, ldsw)
 
    if (small_int) {
-      if (OP == kSAD && bits8 && fits32 && tw >= 16) {
+      if (OP == kSAD && bits8 && fits32 && sad8_blocks(tw) > 0) {
          // the packed template blocks of the tile's AbsDistance loop (in the staged template,
          // which the next call stages again)
          KernelTimer kt(s, "template_prep");
-         const int items = (tw / 16) * th;
+         const int items = sad8_blocks(tw) * th;
          int32_t *tp = static_cast<int32_t *>(const_cast<void *>(tstage));
          OCIP_TYPE_SWITCH(S.type, TT, {
             if constexpr (TT == SImage::U8 || TT == SImage::S8)

@@ -90,7 +90,7 @@ def test_f32_norm(ocip, W, H, tw, th):
 
 
 @pytest.mark.parametrize("dtype", [np.uint8, np.int8, np.uint16, np.int16])
-@pytest.mark.parametrize("tw,th", [(3, 3), (16, 16), (32, 32), (9, 20), (40, 7)])
+@pytest.mark.parametrize("tw,th", [(3, 3), (16, 16), (32, 32), (9, 20), (40, 7), (20, 5), (35, 4)])
 def test_integer_exact(ocip, dtype, tw, th):
     rng = np.random.default_rng(tw * th)
     src = rand(rng, (70, 90), dtype)
