@@ -602,11 +602,31 @@ __global__ __launch_bounds__(256) void prox_tile_int(const char *src, int W, int
 #pragma unroll
                for (int r = 0; r < R; ++r) acc[r] = itap<TOP, ACC>(acc[r], sw[r + k], t[k]);
          }
-         for (; u0 < tw; ++u0) {
-            const int32_t t = sad_key<TOP, TYPE, ACC>(trow[u0]);
+         if (u0 < tw) {
+            // the last tw % kIntU taps (1 to kIntU - 1) from 16-byte reads of the aligned column
+            // u0 too (4-byte LDS reads of consecutive rows would meet in the same banks); the
+            // reads end at most at column colbase + R + tw + 1 <= ncols + 2, inside the row
+            // pitch
+            const int rem = tw - u0;
+            constexpr int NQ = (R + kIntU - 1 + 3) / 4;
+            int32_t t[kIntU - 1], sw[4 * NQ];
+            const int4 *w4 = reinterpret_cast<const int4 *>(lrow + u0);
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-               acc[r] = itap<TOP, ACC>(acc[r], sad_key<TOP, TYPE, ACC>(lrow[u0 + r]), t);
+            for (int m = 0; m < NQ; ++m) {
+               int4 v4 = make_int4(0, 0, 0, 0);
+               if (4 * m < R + rem - 1) v4 = w4[m];
+               sw[4 * m] = v4.x; sw[4 * m + 1] = v4.y; sw[4 * m + 2] = v4.z; sw[4 * m + 3] = v4.w;
+            }
+#pragma unroll
+            for (int j = 0; j < R + kIntU - 2; ++j) sw[j] = sad_key<TOP, TYPE, ACC>(sw[j]);
+#pragma unroll
+            for (int k = 0; k < kIntU - 1; ++k) t[k] = k < rem ? sad_key<TOP, TYPE, ACC>(trow[u0 + k]) : 0;
+#pragma unroll
+            for (int k = 0; k < kIntU - 1; ++k) {
+               if (k >= rem) break;
+#pragma unroll
+               for (int r = 0; r < R; ++r) acc[r] = itap<TOP, ACC>(acc[r], sw[r + k], t[k]);
+            }
          }
       }
    }
