@@ -269,13 +269,13 @@ __global__ __launch_bounds__(256) void prox_tile_f32(const char *src, int W, int
          for (int c = 0; c < nfull; ++c, u0 += U)
             chunk_f32<OP, R, U, true>(acc, lrow, colbase + u0, trow + u0);
          if (rem & 4) { chunk_f32<OP, R, 4, true>(acc, lrow, colbase + u0, trow + u0); u0 += 4; }
-         // the last rem % 4 taps in one chunk, from a 16-byte aligned column too (4-byte LDS
-         // reads of consecutive rows would meet in the same banks): its 16-byte reads end at
-         // most at column ncols + 2, inside the row pitch
-         switch (rem & 3) {
-         case 1: chunk_f32<OP, R, 1, true>(acc, lrow, colbase + u0, trow + u0); break;
-         case 2: chunk_f32<OP, R, 2, true>(acc, lrow, colbase + u0, trow + u0); break;
-         case 3: chunk_f32<OP, R, 3, true>(acc, lrow, colbase + u0, trow + u0); break;
+         // the 2- and 1-column chunks from 16-byte reads where their column is 16-byte
+         // aligned (4-byte LDS reads of consecutive rows would meet in the same banks): the
+         // reads end at most at column ncols + 2, inside the row pitch
+         if (rem & 2) { chunk_f32<OP, R, 2, true>(acc, lrow, colbase + u0, trow + u0); u0 += 2; }
+         if (rem & 1) {
+            if (rem & 2) chunk_f32<OP, R, 1, false>(acc, lrow, colbase + u0, trow + u0);
+            else chunk_f32<OP, R, 1, true>(acc, lrow, colbase + u0, trow + u0);
          }
       }
    }
