@@ -72,22 +72,23 @@ template <int OP> __device__ __forceinline__ float tap(float acc, float s, float
 }
 
 // U template columns starting at LDS column `col` (row pointer `lrow`) for R outputs.
-// ALIGNED: col is a multiple of 4 floats, so the window is read with 16-byte LDS loads.
-template <int OP, int R, int U, bool ALIGNED>
+// ALIGNED: col - SHIFT is a multiple of 4 floats, so the window is read with 16-byte LDS
+// loads from there (SHIFT values before col read and not used).
+template <int OP, int R, int U, bool ALIGNED, int SHIFT = 0>
 __device__ __forceinline__ void chunk_f32(float (&acc)[R], const float *lrow, int col,
                                           const float *__restrict__ trow)
 {
    constexpr int N = R + U - 1;
-   float s[(N + 3) / 4 * 4];
+   float s[(N + SHIFT + 3) / 4 * 4];
    if (ALIGNED) {
 #pragma unroll
-      for (int i = 0; i < (N + 3) / 4; ++i) {
-         float4 q = *reinterpret_cast<const float4 *>(lrow + col + 4 * i);
+      for (int i = 0; i < (N + SHIFT + 3) / 4; ++i) {
+         float4 q = *reinterpret_cast<const float4 *>(lrow + col - SHIFT + 4 * i);
          s[4 * i] = q.x; s[4 * i + 1] = q.y; s[4 * i + 2] = q.z; s[4 * i + 3] = q.w;
       }
    } else {
 #pragma unroll
-      for (int i = 0; i < N; ++i) s[i] = lrow[col + i];
+      for (int i = 0; i < N; ++i) s[i + SHIFT] = lrow[col + i];
    }
    float t[U];
 #pragma unroll
@@ -95,7 +96,7 @@ __device__ __forceinline__ void chunk_f32(float (&acc)[R], const float *lrow, in
 #pragma unroll
    for (int k = 0; k < U; ++k)
 #pragma unroll
-      for (int r = 0; r < R; ++r) acc[r] = tap<OP>(acc[r], s[r + k], t[k]);
+      for (int r = 0; r < R; ++r) acc[r] = tap<OP>(acc[r], s[r + k + SHIFT], t[k]);
 }
 
 template <int TYPE>
@@ -269,12 +270,13 @@ __global__ __launch_bounds__(256) void prox_tile_f32(const char *src, int W, int
          for (int c = 0; c < nfull; ++c, u0 += U)
             chunk_f32<OP, R, U, true>(acc, lrow, colbase + u0, trow + u0);
          if (rem & 4) { chunk_f32<OP, R, 4, true>(acc, lrow, colbase + u0, trow + u0); u0 += 4; }
-         // the 2- and 1-column chunks from 16-byte reads where their column is 16-byte
-         // aligned (4-byte LDS reads of consecutive rows would meet in the same banks): the
-         // reads end at most at column ncols + 2, inside the row pitch
+         // the 2- and 1-column chunks from 16-byte reads of a 16-byte aligned column (4-byte
+         // LDS reads of consecutive rows would meet in the same banks), the 1-column chunk after
+         // a 2-column one from two columns before its own: the reads end at most at column
+         // ncols + 2, inside the row pitch
          if (rem & 2) { chunk_f32<OP, R, 2, true>(acc, lrow, colbase + u0, trow + u0); u0 += 2; }
          if (rem & 1) {
-            if (rem & 2) chunk_f32<OP, R, 1, false>(acc, lrow, colbase + u0, trow + u0);
+            if (rem & 2) chunk_f32<OP, R, 1, true, 2>(acc, lrow, colbase + u0, trow + u0);
             else chunk_f32<OP, R, 1, true>(acc, lrow, colbase + u0, trow + u0);
          }
       }
