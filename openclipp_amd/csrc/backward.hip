@@ -112,7 +112,7 @@ struct BoxEpi {
 size_t box_f64_lds(int tw, int th)
 {
    const size_t ncols = kBxCols - 1 + tw, nrows = kBxRows - 1 + th;
-   return (size_t)kBxRows * ncols * sizeof(double) + nrows * ncols * sizeof(float);
+   return (size_t)kBxRows * (ncols | 1) * sizeof(double) + nrows * ncols * sizeof(float);
 }
 
 template <int EPI>
@@ -126,8 +126,11 @@ __global__ __launch_bounds__(kThreads) void box_f64(ZPtrs maps, int W, int H, in
    const char *m = maps.p[z];
    const int mstep = maps.step[z];
    const int ncols = kBxCols - 1 + tw, nrows = kBxRows - 1 + th;
-   double *vs = bx_lds;                                            // kBxRows x ncols
-   float *ms = reinterpret_cast<float *>(vs + kBxRows * ncols);   // nrows x ncols
+   // column sums V at an odd pitch of doubles: the horizontal pass's lanes (eight rows by
+   // eight 8-column groups per wavefront) then read 16 distinct pairs of LDS banks, not 8
+   const int vp = ncols | 1;
+   double *vs = bx_lds;                                            // kBxRows x vp
+   float *ms = reinterpret_cast<float *>(vs + kBxRows * vp);      // nrows x ncols
    const int x0 = (blockIdx.x % tiles_x) * kBxCols, y0 = (blockIdx.x / tiles_x) * kBxRows;
    const int c0 = x0 - ax, r0 = y0 - ay;
    const int total = nrows * ncols;
@@ -160,7 +163,7 @@ __global__ __launch_bounds__(kThreads) void box_f64(ZPtrs maps, int W, int H, in
       for (int i = 0; i < th - 1; ++i) lead += fv(i, c);
       for (int r = 0; r < kBxRows; ++r) {
          lead += fv(r + th - 1, c);
-         vs[r * ncols + c] = lead - lag;
+         vs[r * vp + c] = lead - lag;
          lag += fv(r, c);
       }
    }
@@ -170,7 +173,7 @@ __global__ __launch_bounds__(kThreads) void box_f64(ZPtrs maps, int W, int H, in
    const size_t zo = (size_t)z * ep.zstride;
    double a1 = 0.0, a2 = 0.0;
    if (y < H) {
-      const double *vrow = vs + r * ncols + q;
+      const double *vrow = vs + r * vp + q;
       double lead = 0.0, lag = 0.0;
       for (int k = 0; k < tw - 1; ++k) lead += vrow[k];
       const size_t ro = zo + (size_t)y * W;
