@@ -104,17 +104,25 @@ __device__ __forceinline__ void stage_tile_f32(float *lds, int ldsw, int nrows, 
                                                const char *src, int W, int H, int sstep,
                                                int nch, int ch, int sx0, int sy0)
 {
-   // eight elements per thread in flight
+   // eight elements per thread in flight; element i = r ldsw + c, (r, c) stepped along with i
+   // (256 elements a step) instead of divided out for each element
    const int n = nrows * ldsw;
+   const int dq = 256 / ldsw, dr = 256 - dq * ldsw;
+   int r = threadIdx.x / ldsw, c = threadIdx.x - r * ldsw;
    for (int i0 = threadIdx.x; i0 < n; i0 += 256 * 8) {
       float v[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
          const int i = i0 + 256 * k;
-         const int r = i / ldsw, c = i - r * ldsw;
          const int sx = sx0 + c, sy = sy0 + r;
          v[k] = (i < n && c < ncols && sx >= 0 && sx < W && sy >= 0 && sy < H)
                     ? (float)ld<TYPE>(src + (size_t)sy * sstep, sx * nch + ch) : 0.f;
+         c += dr;
+         r += dq;
+         if (c >= ldsw) {
+            c -= ldsw;
+            ++r;
+         }
       }
 #pragma unroll
       for (int k = 0; k < 8; ++k)
@@ -315,15 +323,24 @@ __device__ __forceinline__ void stage_tile_int(int32_t *lds, int ldsw, int nrows
                                                int nch, int ch, int sx0, int sy0)
 {
    const int n = nrows * ldsw;
+   // element i = r ldsw + c, (r, c) stepped along with i (256 elements a step) instead of
+   // divided out for each element
+   const int dq = 256 / ldsw, dr = 256 - dq * ldsw;
+   int r = threadIdx.x / ldsw, c = threadIdx.x - r * ldsw;
    for (int i0 = threadIdx.x; i0 < n; i0 += 256 * 8) {
       int32_t v[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
          const int i = i0 + 256 * k;
-         const int r = i / ldsw, c = i - r * ldsw;
          const int sx = sx0 + c, sy = sy0 + r;
          v[k] = (i < n && c < ncols && sx >= 0 && sx < W && sy >= 0 && sy < H)
                     ? (int32_t)ld<TYPE>(src + (size_t)sy * sstep, sx * nch + ch) : 0;
+         c += dr;
+         r += dq;
+         if (c >= ldsw) {
+            c -= ldsw;
+            ++r;
+         }
       }
 #pragma unroll
       for (int k = 0; k < 8; ++k)
@@ -388,23 +405,29 @@ template <int TYPE>
 __device__ __forceinline__ void pack_sad8(int32_t *lds, int ldsw, int nrows, int ncols)
 {
    const int n = nrows * ldsw;
+   // column c = i % ldsw stepped along with i (256 elements a step)
+   const int dr = 256 % ldsw;
+   int c = threadIdx.x % ldsw;
    for (int i0 = 0; i0 < n; i0 += 256 * 8) {
       uint32_t p[8];
+      bool in[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
          const int i = i0 + threadIdx.x + 256 * k;
-         const int c = i % ldsw;
+         in[k] = i < n && c < ncols;
          p[k] = 0;
-         if (i < n && c < ncols)
+         if (in[k])
             p[k] = pack4(sad_key<kSAD, TYPE, int32_t>(lds[i]), sad_key<kSAD, TYPE, int32_t>(lds[i + 4]),
                          sad_key<kSAD, TYPE, int32_t>(lds[i + 8]),
                          sad_key<kSAD, TYPE, int32_t>(lds[i + 12]));
+         c += dr;
+         if (c >= ldsw) c -= ldsw;
       }
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
          const int i = i0 + threadIdx.x + 256 * k;
-         if (i < n && i % ldsw < ncols) lds[i] = (int32_t)p[k];
+         if (in[k]) lds[i] = (int32_t)p[k];
       }
       __syncthreads();
    }
