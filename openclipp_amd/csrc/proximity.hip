@@ -139,12 +139,17 @@ __device__ __forceinline__ void stage_tile_f32_v4(float *lds, int ldsw, int nrow
                                                   int sx0, int sy0)
 {
    const int gpr = ldsw / 4, n4 = nrows * gpr;
+   // group i = r gpr + g, (r, g) stepped along with i (256 groups a step) instead of divided out
+   const int dq = 256 / gpr, dr = 256 - dq * gpr;
+   int r = threadIdx.x / gpr, g = threadIdx.x - r * gpr;
    for (int i0 = threadIdx.x; i0 < n4; i0 += 256 * 8) {
       float4 v[8];
+      int at[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
          const int i = i0 + 256 * k;
-         const int r = i / gpr, c = 4 * (i - r * gpr);
+         const int c = 4 * g;
+         at[k] = r * ldsw + c;
          const int sx = sx0 + c, sy = sy0 + r;
          const bool rowin = i < n4 && sy >= 0 && sy < H;
          const float *srow = reinterpret_cast<const float *>(src + (size_t)(rowin ? sy : 0) * sstep);
@@ -157,13 +162,18 @@ __device__ __forceinline__ void stage_tile_f32_v4(float *lds, int ldsw, int nrow
                e[j] = (rowin && c + j < ncols && sx + j >= 0 && sx + j < W) ? srow[sx + j] : 0.f;
             v[k] = make_float4(e[0], e[1], e[2], e[3]);
          }
+         g += dr;
+         r += dq;
+         if (g >= gpr) {
+            g -= gpr;
+            ++r;
+         }
       }
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
          const int i = i0 + 256 * k;
          if (i >= n4) break;
-         const int r = i / gpr, c = 4 * (i - r * gpr);
-         *reinterpret_cast<float4 *>(lds + (size_t)r * ldsw + c) = v[k];
+         *reinterpret_cast<float4 *>(lds + at[k]) = v[k];
       }
    }
 }
